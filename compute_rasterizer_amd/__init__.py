@@ -1,0 +1,201 @@
+"""compute_rasterizer_amd — MI355X-native point-cloud software rasterizer.
+
+Python face of libpcr.so (include/pcr.h). The per-frame call pair
+`Context.render` + `Context.resolve` stands in for the reference's
+Method::render (include/Method.h:10-24, compute_loop_las.h:117-304).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import (BATCH_DTYPE, BG_BLACK, BG_LAS, FB_CLEAR, FB_HQS, FB_KEY64, LOD_SIZE80, LOD_SPACING,
+                   MODE_HQS, MODE_HQS16, MODE_LAS, MODE_LAS2, MODE_NODES, MODE_PLAIN16, POINTS_PER_BATCH,
+                   SCENE_CITY, SCENE_CUBE, SCENE_TERRAIN, BuildOpts, GenCfg, OctreeOpts, PcrError,
+                   RenderParams, Stats, Tune, View, check, ptr)
+
+__all__ = ["Context", "Cloud", "Framebuffer", "look_at", "params", "PcrError", "View", "RenderParams",
+           "MODE_PLAIN16", "MODE_LAS", "MODE_LAS2", "MODE_HQS", "MODE_NODES", "MODE_HQS16",
+           "LOD_SPACING", "LOD_SIZE80", "SCENE_CUBE", "SCENE_TERRAIN", "SCENE_CITY", "FB_KEY64", "FB_HQS",
+           "BG_LAS", "BG_BLACK", "FB_CLEAR", "POINTS_PER_BATCH", "BATCH_DTYPE"]
+
+
+def look_at(eye, target, up=(0.0, 1.0, 0.0), fovy=60.0, near=0.1, far=1000.0, origin=(0.0, 0.0, 0.0),
+            width=1280, height=720) -> View:
+    """Camera -> pcr_view (double-precision glm lookAt/perspective, cast to float)."""
+    v = View()
+    check(capi.load().pcr_view_look_at(capi.dvec(eye), capi.dvec(target), capi.dvec(up), fovy, near, far,
+                                       capi.dvec(origin), width, height, C.byref(v)))
+    return v
+
+
+def params(mode=MODE_LAS2, culling=1, background=None, lod_rule=LOD_SPACING, lod_px=1.0, block_size=0,
+           clear_before=1, time_stages=0) -> RenderParams:
+    if background is None:
+        background = BG_BLACK if mode in (MODE_PLAIN16,) else BG_LAS
+    return RenderParams(mode, culling, lod_rule, lod_px, background, block_size, clear_before, time_stages)
+
+
+def gen_cfg(scene, n, seed=1, extent=2000.0, amplitude=150.0, jitter=0.5, num_buildings=256,
+            terrain_frac=0.2) -> GenCfg:
+    return GenCfg(scene, num_buildings, seed, n, extent, amplitude, jitter, terrain_frac)
+
+
+def encode_batches_host(xyz: np.ndarray, batch_size: int = POINTS_PER_BATCH):
+    """Host encoder (LasLoaderSparse.cpp:37-212): returns (batches, lo, med, hi)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+    n = xyz.shape[0]
+    nb = (n + batch_size - 1) // batch_size
+    b = np.zeros(nb, BATCH_DTYPE)
+    lo, med, hi = (np.zeros(n, np.uint32) for _ in range(3))
+    check(capi.load().pcr_encode_batches_host(ptr(xyz), n, batch_size, ptr(b), ptr(lo), ptr(med), ptr(hi)))
+    return b, lo, med, hi
+
+
+class Framebuffer:
+    def __init__(self, ctx: "Context", width: int, height: int, kind: int = FB_KEY64):
+        self.ctx, self.width, self.height, self.kind = ctx, width, height, kind
+        h = C.c_void_p()
+        check(capi.load().pcr_fb_create(ctx.h, width, height, kind, C.byref(h)))
+        self.h = h
+
+    @property
+    def npix(self):
+        return self.width * self.height
+
+    def clear(self):
+        check(capi.load().pcr_fb_clear(self.ctx.h, self.h))
+
+    def keys(self) -> np.ndarray:
+        out = np.empty(self.npix, np.uint64)
+        check(capi.load().pcr_fb_readback(self.ctx.h, self.h, ptr(out)))
+        return out
+
+    def hqs(self):
+        d = np.empty(self.npix, np.uint32)
+        a = np.empty(4 * self.npix, np.uint32)
+        check(capi.load().pcr_fb_readback_hqs(self.ctx.h, self.h, ptr(d), ptr(a)))
+        return d, a
+
+    def rgba(self) -> np.ndarray:
+        out = np.empty(self.npix, np.uint32)
+        check(capi.load().pcr_fb_readback_rgba(self.ctx.h, self.h, ptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            capi.load().pcr_fb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Cloud:
+    def __init__(self, ctx: "Context", handle: C.c_void_p):
+        self.ctx, self.h = ctx, handle
+
+    def info(self) -> capi.CloudInfo:
+        i = capi.CloudInfo()
+        check(capi.load().pcr_cloud_info_get(self.h, C.byref(i)))
+        return i
+
+    def download(self, batched=True, points16=False):
+        i = self.info()
+        out = {}
+        if batched and i.has_batches:
+            out["batches"] = np.zeros(i.num_batches, BATCH_DTYPE)
+            for k in ("lo", "med", "hi", "rgba"):
+                out[k] = np.zeros(i.num_points, np.uint32)
+        if points16 and i.has_points16:
+            out["xyz16"] = np.zeros((i.num_points, 3), np.float32)
+            out["rgba16"] = np.zeros(i.num_points, np.uint32)
+        check(capi.load().pcr_cloud_download(self.h, ptr(out.get("batches")), ptr(out.get("lo")),
+                                             ptr(out.get("med")), ptr(out.get("hi")), ptr(out.get("rgba")),
+                                             ptr(out.get("xyz16")), ptr(out.get("rgba16"))))
+        return out
+
+    def close(self):
+        if self.h:
+            capi.load().pcr_cloud_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP device + stream (pcr_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(capi.load().pcr_create(device, C.byref(h)))
+        self.h = h
+
+    # --- clouds
+    def upload_points16(self, xyz, rgba) -> Cloud:
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        rgba = np.ascontiguousarray(rgba, np.uint32)
+        h = C.c_void_p()
+        check(capi.load().pcr_cloud_upload_points16(self.h, ptr(xyz), ptr(rgba), xyz.shape[0], C.byref(h)))
+        return Cloud(self, h)
+
+    def upload_batched(self, batches, lo, med, hi, rgba) -> Cloud:
+        batches = np.ascontiguousarray(batches, BATCH_DTYPE)
+        arrs = [np.ascontiguousarray(a, np.uint32) for a in (lo, med, hi, rgba)]
+        h = C.c_void_p()
+        check(capi.load().pcr_cloud_upload_batched(self.h, ptr(batches), len(batches), *[ptr(a) for a in arrs],
+                                                   arrs[0].shape[0], C.byref(h)))
+        return Cloud(self, h)
+
+    def generate(self, cfg: GenCfg, batch_size=POINTS_PER_BATCH, morton_sort=True, keep_points16=False) -> Cloud:
+        h = C.c_void_p()
+        opts = BuildOpts(batch_size, int(morton_sort), int(keep_points16))
+        check(capi.load().pcr_cloud_generate(self.h, C.byref(cfg), C.byref(opts), C.byref(h)))
+        return Cloud(self, h)
+
+    def octree_host(self, xyz, rgba, seed=1, root_spacing_div=128.0, max_node_points=20000, max_depth=12) -> Cloud:
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        rgba = np.ascontiguousarray(rgba, np.uint32)
+        h = C.c_void_p()
+        o = OctreeOpts(seed, root_spacing_div, max_node_points, max_depth)
+        check(capi.load().pcr_cloud_build_octree_host(self.h, ptr(xyz), ptr(rgba), xyz.shape[0], C.byref(o),
+                                                      C.byref(h)))
+        return Cloud(self, h)
+
+    def framebuffer(self, width, height, kind=FB_KEY64) -> Framebuffer:
+        return Framebuffer(self, width, height, kind)
+
+    # --- frame
+    def render(self, cloud: Cloud, view: View, prm: RenderParams, fb: Framebuffer):
+        check(capi.load().pcr_render(self.h, cloud.h, C.byref(view), C.byref(prm), fb.h))
+
+    def resolve(self, cloud: Cloud, prm: RenderParams, fb: Framebuffer):
+        check(capi.load().pcr_resolve(self.h, cloud.h, C.byref(prm), fb.h))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(capi.load().pcr_stats_get(self.h, C.byref(s)))
+        return s
+
+    def lod_selection(self, cloud: Cloud) -> np.ndarray:
+        n = cloud.info().num_batches
+        out = np.zeros(n, np.uint8)
+        check(capi.load().pcr_lod_selection(self.h, cloud.h, ptr(out), n))
+        return out
+
+    def synchronize(self):
+        check(capi.load().pcr_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            capi.load().pcr_destroy(self.h)
+            self.h = None

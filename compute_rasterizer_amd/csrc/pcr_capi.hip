@@ -1,0 +1,466 @@
+// pcr_capi.hip — extern "C" entry points of libpcr.so (declared in include/pcr.h).
+//
+// The per-frame sequence of pcr_render + pcr_resolve stands in for
+// ComputeLoopLas::render (modules/compute_loop_las/compute_loop_las.h:117-304):
+// clear -> select (cull + precision) -> raster -> resolve, all queued on the
+// context's HIP stream. Errors are status codes plus pcr_last_error(), where
+// the reference silently skipped (compute_loop_las.h:123,210,241).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include "pcr_internal.h"
+
+namespace pcr {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    return e == hipErrorOutOfMemory ? PCR_E_OOM : PCR_E_HIP;
+}
+
+static bool valid_view(const pcr_view* v) {
+    return v && v->width > 0 && v->height > 0 && (uint64_t)v->width * (uint64_t)v->height < (1ull << 32);
+}
+
+static int clear_fb(hipStream_t s, pcr_fb* fb) {
+    const uint64_t npix = (uint64_t)fb->width * fb->height;
+    if (fb->kind == PCR_FB_KEY64) return launch_fill64(s, fb->d_keys, npix, PCR_FB_CLEAR);
+    int rc = launch_fill32(s, fb->d_depth, npix, PCR_DEPTH_CLEAR);
+    if (rc) return rc;
+    return launch_fill32(s, fb->d_acc, 4 * npix, 0u);
+}
+
+}  // namespace pcr
+
+using namespace pcr;
+
+extern "C" {
+
+const char* pcr_last_error(void) { return g_err; }
+int pcr_version(void) { return PCR_VERSION_MAJOR * 100 + PCR_VERSION_MINOR; }
+
+int pcr_device_count(int* count) {
+    PCR_CHECK(count, "device_count: null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) { (void)hipGetLastError(); n = 0; }
+    *count = n;
+    return PCR_OK;
+}
+
+int pcr_create(int device, pcr_ctx** out) {
+    PCR_CHECK(out, "create: null out");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device available");
+        return PCR_E_NO_DEVICE;
+    }
+    PCR_CHECK(device >= 0 && device < n, "create: device %d out of range (%d devices)", device, n);
+    PCR_HIP(hipSetDevice(device));
+    pcr_ctx* c = new pcr_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    PCR_HIP(hipGetDeviceProperties(&prop, device));
+    c->cu_count = prop.multiProcessorCount;
+    PCR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    PCR_HIP(hipMalloc(&c->d_state, sizeof(FrameState)));
+    PCR_HIP(hipMemset(c->d_state, 0, sizeof(FrameState)));
+    for (auto& e : c->ev) PCR_HIP(hipEventCreate(&e));
+    *out = c;
+    return PCR_OK;
+}
+
+int pcr_destroy(pcr_ctx* ctx) {
+    if (!ctx) return PCR_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+    (void)hipFree(ctx->d_state);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PCR_OK;
+}
+
+int pcr_synchronize(pcr_ctx* ctx) {
+    PCR_CHECK(ctx, "synchronize: null ctx");
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    return PCR_OK;
+}
+
+void* pcr_stream(pcr_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+// glm lookAtRH / perspectiveRH_NO in double (Camera.h:34-38,
+// matrix_transform.inl:258-274); world = translate(origin)
+// (compute_loop_las.h:170-176); the products are cast to float.
+int pcr_view_look_at(const double eye[3], const double target[3], const double up[3], double fovy_deg,
+                     double near_plane, double far_plane, const double origin[3], int width, int height,
+                     pcr_view* out) {
+    PCR_CHECK(eye && target && up && origin && out, "view_look_at: null argument");
+    PCR_CHECK(width > 0 && height > 0, "view_look_at: bad size %dx%d", width, height);
+    PCR_CHECK(near_plane > 0 && far_plane > near_plane, "view_look_at: bad clip planes");
+    double f[3] = {target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]};
+    const double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    PCR_CHECK(fl > 0, "view_look_at: eye == target");
+    for (double& x : f) x /= fl;
+    double s[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+    const double sl = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    PCR_CHECK(sl > 0, "view_look_at: up parallel to view direction");
+    for (double& x : s) x /= sl;
+    const double u[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2], s[0] * f[1] - s[1] * f[0]};
+    double V[16] = {0}, P[16] = {0}, W[16] = {0}, VW[16], PVW[16];
+    V[0] = s[0]; V[4] = s[1]; V[8] = s[2];
+    V[1] = u[0]; V[5] = u[1]; V[9] = u[2];
+    V[2] = -f[0]; V[6] = -f[1]; V[10] = -f[2];
+    V[12] = -(s[0] * eye[0] + s[1] * eye[1] + s[2] * eye[2]);
+    V[13] = -(u[0] * eye[0] + u[1] * eye[1] + u[2] * eye[2]);
+    V[14] = (f[0] * eye[0] + f[1] * eye[1] + f[2] * eye[2]);
+    V[15] = 1.0;
+    const double th = std::tan(fovy_deg * 3.14159265358979323846 / 180.0 / 2.0);
+    const double aspect = (double)width / (double)height;
+    P[0] = 1.0 / (aspect * th);
+    P[5] = 1.0 / th;
+    P[10] = -(far_plane + near_plane) / (far_plane - near_plane);
+    P[11] = -1.0;
+    P[14] = -(2.0 * far_plane * near_plane) / (far_plane - near_plane);
+    W[0] = W[5] = W[10] = W[15] = 1.0;
+    W[12] = origin[0]; W[13] = origin[1]; W[14] = origin[2];
+    auto mul = [](const double* a, const double* b, double* r) {
+        for (int c = 0; c < 4; c++)
+            for (int rr = 0; rr < 4; rr++) {
+                double acc = 0.0;
+                for (int k = 0; k < 4; k++) acc += a[k * 4 + rr] * b[c * 4 + k];
+                r[c * 4 + rr] = acc;
+            }
+    };
+    mul(V, W, VW);
+    mul(P, VW, PVW);
+    memset(out, 0, sizeof(*out));
+    for (int i = 0; i < 16; i++) {
+        out->transform[i] = (float)PVW[i];
+        out->transform_frustum[i] = (float)PVW[i];
+        out->view_world[i] = (float)VW[i];
+        out->proj[i] = (float)P[i];
+    }
+    out->width = width;
+    out->height = height;
+    return PCR_OK;
+}
+
+int pcr_encode_batches_host(const float* xyz, uint64_t n, uint32_t batch_size, pcr_batch* batches,
+                            uint32_t* lo, uint32_t* med, uint32_t* hi) {
+    PCR_CHECK(xyz && batches && lo && med && hi, "encode_batches_host: null argument");
+    PCR_CHECK(batch_size > 0 && batch_size % 4 == 0, "encode_batches_host: batch_size %u not a multiple of 4",
+              batch_size);
+    PCR_CHECK(n < (1ull << 32), "encode_batches_host: more than 2^32-1 points");
+    encode_batches_host(xyz, n, batch_size, batches, lo, med, hi);
+    return PCR_OK;
+}
+
+// ------------------------------------------------------------------ clouds
+int pcr_cloud_upload_points16(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, uint64_t n, pcr_cloud** out) {
+    PCR_CHECK(ctx && xyz && rgba && out, "upload_points16: null argument");
+    PCR_CHECK(n > 0, "upload_points16: no points");
+    std::vector<uint32_t> packed(4 * n);
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint64_t i = 0; i < n; i++) {
+        memcpy(&packed[4 * i], xyz + 3 * i, 12);
+        packed[4 * i + 3] = rgba[i];
+        for (int a = 0; a < 3; a++) {
+            const float v = xyz[3 * i + a];
+            mn[a] = v < mn[a] ? v : mn[a];
+            mx[a] = v > mx[a] ? v : mx[a];
+        }
+    }
+    pcr_cloud* c = new pcr_cloud();
+    c->ctx = ctx;
+    c->num_points = n;
+    c->points16_len = n;
+    if (hipMalloc(&c->d_points16, n * 16) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        set_error("upload_points16: device allocation of %llu bytes failed", (unsigned long long)(n * 16));
+        return PCR_E_OOM;
+    }
+    c->device_bytes = n * 16;
+    for (int a = 0; a < 3; a++) { c->aabb_min[a] = mn[a]; c->aabb_max[a] = mx[a]; }
+    PCR_HIP(hipMemcpy(c->d_points16, packed.data(), n * 16, hipMemcpyHostToDevice));
+    *out = c;
+    return PCR_OK;
+}
+
+int pcr_cloud_upload_batched(pcr_ctx* ctx, const pcr_batch* batches, uint32_t nb, const uint32_t* lo,
+                             const uint32_t* med, const uint32_t* hi, const uint32_t* rgba, uint64_t npoints,
+                             pcr_cloud** out) {
+    PCR_CHECK(ctx && batches && lo && med && hi && rgba && out, "upload_batched: null argument");
+    PCR_CHECK(nb > 0 && npoints > 0 && npoints < (1ull << 32), "upload_batched: bad sizes");
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int max_level = -1;
+    for (uint32_t b = 0; b < nb; b++) {
+        const pcr_batch& h = batches[b];
+        PCR_CHECK(h.first_point % 4 == 0, "upload_batched: batch %u first_point %u is not a multiple of 4", b,
+                  h.first_point);
+        PCR_CHECK((uint64_t)h.first_point + h.num_points <= npoints,
+                  "upload_batched: batch %u range [%u, +%u) exceeds %llu points", b, h.first_point, h.num_points,
+                  (unsigned long long)npoints);
+        PCR_CHECK(h.parent < (int32_t)nb && h.parent < (int32_t)b,
+                  "upload_batched: batch %u parent %d must precede it", b, h.parent);
+        for (int a = 0; a < 3; a++) {
+            mn[a] = h.min[a] < mn[a] ? h.min[a] : mn[a];
+            mx[a] = h.max[a] > mx[a] ? h.max[a] : mx[a];
+        }
+        if (h.parent >= 0 || h.spacing > 0) max_level = (int)h.level > max_level ? (int)h.level : max_level;
+    }
+    pcr_cloud* c = new pcr_cloud();
+    int rc = cloud_alloc_batched(ctx, nb, npoints, c);
+    if (rc) { delete c; return rc; }
+    c->batch_size = batches[0].num_points;
+    c->num_levels = max_level + 1;
+    for (int a = 0; a < 3; a++) { c->aabb_min[a] = mn[a]; c->aabb_max[a] = mx[a]; }
+    PCR_HIP(hipMemcpy(c->d_batches, batches, sizeof(pcr_batch) * nb, hipMemcpyHostToDevice));
+    PCR_HIP(hipMemcpy(c->d_lo, lo, npoints * 4, hipMemcpyHostToDevice));
+    PCR_HIP(hipMemcpy(c->d_med, med, npoints * 4, hipMemcpyHostToDevice));
+    PCR_HIP(hipMemcpy(c->d_hi, hi, npoints * 4, hipMemcpyHostToDevice));
+    PCR_HIP(hipMemcpy(c->d_rgba, rgba, npoints * 4, hipMemcpyHostToDevice));
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    c->h_batches.assign(batches, batches + nb);
+    *out = c;
+    return PCR_OK;
+}
+
+int pcr_generate_points(pcr_ctx* ctx, const pcr_gen_cfg* cfg, float* xyz_dev, uint32_t* rgba_dev) {
+    PCR_CHECK(ctx, "generate_points: null ctx");
+    return generate_points(ctx, cfg, xyz_dev, rgba_dev);
+}
+
+int pcr_cloud_build_device(pcr_ctx* ctx, const float* xyz_dev, const uint32_t* rgba_dev, uint64_t n,
+                           const pcr_build_opts* opts, pcr_cloud** out) {
+    return build_cloud_device(ctx, xyz_dev, rgba_dev, n, opts, out);
+}
+
+int pcr_cloud_generate(pcr_ctx* ctx, const pcr_gen_cfg* cfg, const pcr_build_opts* opts, pcr_cloud** out) {
+    PCR_CHECK(ctx && cfg && out, "cloud_generate: null argument");
+    PCR_CHECK(cfg->num_points > 0, "cloud_generate: no points");
+    float* xyz = nullptr;
+    uint32_t* rgba = nullptr;
+    PCR_HIP(hipMalloc(&xyz, cfg->num_points * 12));
+    PCR_HIP(hipMalloc(&rgba, cfg->num_points * 4));
+    int rc = generate_points(ctx, cfg, xyz, rgba);
+    if (rc == PCR_OK) rc = build_cloud_device(ctx, xyz, rgba, cfg->num_points, opts, out);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(xyz);
+    (void)hipFree(rgba);
+    return rc;
+}
+
+int pcr_cloud_info_get(const pcr_cloud* c, pcr_cloud_info* out) {
+    PCR_CHECK(c && out, "cloud_info: null argument");
+    memset(out, 0, sizeof(*out));
+    out->num_points = c->num_points;
+    out->num_batches = c->num_batches;
+    out->batch_size = c->batch_size;
+    out->has_points16 = c->d_points16 != nullptr;
+    out->has_batches = c->d_batches != nullptr;
+    out->num_levels = c->num_levels;
+    for (int a = 0; a < 3; a++) { out->aabb_min[a] = c->aabb_min[a]; out->aabb_max[a] = c->aabb_max[a]; }
+    out->device_bytes = c->device_bytes;
+    return PCR_OK;
+}
+
+int pcr_cloud_download(pcr_cloud* c, pcr_batch* batches, uint32_t* lo, uint32_t* med, uint32_t* hi, uint32_t* rgba,
+                       float* xyz16, uint32_t* rgba16) {
+    PCR_CHECK(c, "cloud_download: null cloud");
+    PCR_HIP(hipStreamSynchronize(c->ctx->stream));
+    const uint64_t n = c->num_points;
+    if (c->d_batches) {
+        if (batches) PCR_HIP(hipMemcpy(batches, c->d_batches, sizeof(pcr_batch) * c->num_batches, hipMemcpyDeviceToHost));
+        if (lo) PCR_HIP(hipMemcpy(lo, c->d_lo, n * 4, hipMemcpyDeviceToHost));
+        if (med) PCR_HIP(hipMemcpy(med, c->d_med, n * 4, hipMemcpyDeviceToHost));
+        if (hi) PCR_HIP(hipMemcpy(hi, c->d_hi, n * 4, hipMemcpyDeviceToHost));
+        if (rgba) PCR_HIP(hipMemcpy(rgba, c->d_rgba, n * 4, hipMemcpyDeviceToHost));
+    }
+    if (c->d_points16 && (xyz16 || rgba16)) {
+        std::vector<uint32_t> p(4 * c->points16_len);
+        PCR_HIP(hipMemcpy(p.data(), c->d_points16, c->points16_len * 16, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < c->points16_len; i++) {
+            if (xyz16) memcpy(xyz16 + 3 * i, &p[4 * i], 12);
+            if (rgba16) rgba16[i] = p[4 * i + 3];
+        }
+    }
+    return PCR_OK;
+}
+
+int pcr_cloud_destroy(pcr_cloud* c) {
+    if (!c) return PCR_OK;
+    (void)hipDeviceSynchronize();
+    cloud_free(c);
+    delete c;
+    return PCR_OK;
+}
+
+// ------------------------------------------------------------ framebuffers
+int pcr_fb_create(pcr_ctx* ctx, int width, int height, int kind, pcr_fb** out) {
+    PCR_CHECK(ctx && out, "fb_create: null argument");
+    PCR_CHECK(width > 0 && height > 0 && (uint64_t)width * height < (1ull << 31), "fb_create: bad size %dx%d",
+              width, height);
+    PCR_CHECK(kind == PCR_FB_KEY64 || kind == PCR_FB_HQS, "fb_create: bad kind %d", kind);
+    const uint64_t npix = (uint64_t)width * height;
+    pcr_fb* f = new pcr_fb();
+    f->ctx = ctx; f->width = width; f->height = height; f->kind = kind;
+    hipError_t e = hipMalloc(&f->d_rgba, npix * 4);
+    if (e == hipSuccess) {
+        if (kind == PCR_FB_KEY64) e = hipMalloc(&f->d_keys, npix * 8);
+        else {
+            e = hipMalloc(&f->d_depth, npix * 4);
+            if (e == hipSuccess) e = hipMalloc(&f->d_acc, npix * 16);
+        }
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(f->d_rgba); (void)hipFree(f->d_keys); (void)hipFree(f->d_depth); (void)hipFree(f->d_acc);
+        delete f;
+        set_error("fb_create: device allocation for %dx%d failed", width, height);
+        return PCR_E_OOM;
+    }
+    f->bytes = npix * (kind == PCR_FB_KEY64 ? 12 : 24);
+    int rc = clear_fb(ctx->stream, f);
+    if (rc == PCR_OK) rc = launch_fill32(ctx->stream, f->d_rgba, npix, 0u);
+    if (rc) { pcr_fb_destroy(f); return rc; }
+    *out = f;
+    return PCR_OK;
+}
+
+int pcr_fb_destroy(pcr_fb* f) {
+    if (!f) return PCR_OK;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(f->d_rgba); (void)hipFree(f->d_keys); (void)hipFree(f->d_depth); (void)hipFree(f->d_acc);
+    delete f;
+    return PCR_OK;
+}
+
+int pcr_fb_clear(pcr_ctx* ctx, pcr_fb* fb) {
+    PCR_CHECK(ctx && fb, "fb_clear: null argument");
+    return clear_fb(ctx->stream, fb);
+}
+
+int pcr_fb_readback(pcr_ctx* ctx, pcr_fb* fb, uint64_t* keys) {
+    PCR_CHECK(ctx && fb && keys, "fb_readback: null argument");
+    PCR_CHECK(fb->kind == PCR_FB_KEY64, "fb_readback: not a KEY64 framebuffer");
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    PCR_HIP(hipMemcpy(keys, fb->d_keys, (uint64_t)fb->width * fb->height * 8, hipMemcpyDeviceToHost));
+    return PCR_OK;
+}
+
+int pcr_fb_readback_hqs(pcr_ctx* ctx, pcr_fb* fb, uint32_t* depth, uint32_t* acc) {
+    PCR_CHECK(ctx && fb, "fb_readback_hqs: null argument");
+    PCR_CHECK(fb->kind == PCR_FB_HQS, "fb_readback_hqs: not an HQS framebuffer");
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    const uint64_t npix = (uint64_t)fb->width * fb->height;
+    if (depth) PCR_HIP(hipMemcpy(depth, fb->d_depth, npix * 4, hipMemcpyDeviceToHost));
+    if (acc) PCR_HIP(hipMemcpy(acc, fb->d_acc, npix * 16, hipMemcpyDeviceToHost));
+    return PCR_OK;
+}
+
+int pcr_fb_readback_rgba(pcr_ctx* ctx, pcr_fb* fb, uint32_t* rgba) {
+    PCR_CHECK(ctx && fb && rgba, "fb_readback_rgba: null argument");
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    PCR_HIP(hipMemcpy(rgba, fb->d_rgba, (uint64_t)fb->width * fb->height * 4, hipMemcpyDeviceToHost));
+    return PCR_OK;
+}
+
+void* pcr_fb_device_keys(pcr_fb* fb) { return fb ? (void*)fb->d_keys : nullptr; }
+void* pcr_fb_device_rgba(pcr_fb* fb) { return fb ? (void*)fb->d_rgba : nullptr; }
+
+// ------------------------------------------------------------------ render
+int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_render_params* params, pcr_fb* fb) {
+    PCR_CHECK(ctx && cloud && params && fb, "render: null argument");
+    PCR_CHECK(valid_view(view), "render: bad view");
+    PCR_CHECK(view->width == fb->width && view->height == fb->height,
+              "render: view size %dx%d differs from framebuffer %dx%d", view->width, view->height, fb->width,
+              fb->height);
+    const int mode = params->mode;
+    const bool hqs = mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16;
+    const bool pts16 = mode == PCR_MODE_PLAIN16 || mode == PCR_MODE_HQS16;
+    PCR_CHECK(mode >= PCR_MODE_PLAIN16 && mode <= PCR_MODE_HQS16, "render: unknown mode %d", mode);
+    PCR_CHECK(hqs == (fb->kind == PCR_FB_HQS), "render: mode %d needs a %s framebuffer", mode,
+              hqs ? "HQS" : "KEY64");
+    PCR_CHECK(pts16 ? cloud->d_points16 != nullptr : cloud->d_batches != nullptr,
+              "render: cloud has no %s layout for mode %d", pts16 ? "16 B/pt" : "batched", mode);
+    PCR_CHECK(mode != PCR_MODE_NODES || params->lod_rule == PCR_LOD_SPACING || params->lod_rule == PCR_LOD_SIZE80,
+              "render: bad lod_rule %d", params->lod_rule);
+    const int block = params->block_size;
+    PCR_CHECK(block == 0 || block == 64 || block == 128 || block == 256 || block == 512,
+              "render: block_size %d not in {64,128,256,512}", block);
+    hipStream_t s = ctx->stream;
+    ctx->timed = params->time_stages != 0;
+    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[0], s));
+    int rc = PCR_OK;
+    if (params->clear_before && (rc = clear_fb(s, fb))) return rc;
+    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[1], s));
+    if ((rc = launch_frame_reset(s, ctx->d_state))) return rc;
+    if (!pts16) {
+        rc = launch_select(s, cloud, *view, *params, ctx->d_state, mode == PCR_MODE_NODES ? 80.0f : 100.0f,
+                           mode == PCR_MODE_NODES);
+        if (rc) return rc;
+    }
+    ctx->last_nb = pts16 ? 0 : cloud->num_batches;
+    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[2], s));
+    switch (mode) {
+        case PCR_MODE_PLAIN16: rc = launch_raster_points16(s, cloud, *view, fb->d_keys); break;
+        case PCR_MODE_LAS: rc = launch_raster_batched(s, cloud, *view, 1, block ? block : 128, ctx->d_state, fb->d_keys); break;
+        case PCR_MODE_LAS2:
+        case PCR_MODE_NODES: rc = launch_raster_batched(s, cloud, *view, 4, block ? block : 256, ctx->d_state, fb->d_keys); break;
+        case PCR_MODE_HQS: rc = launch_hqs_batched(s, cloud, *view, ctx->d_state, fb); break;
+        case PCR_MODE_HQS16: rc = launch_hqs_points16(s, cloud, *view, fb); break;
+    }
+    if (rc) return rc;
+    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[3], s));
+    return PCR_OK;
+}
+
+int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params, pcr_fb* fb) {
+    PCR_CHECK(ctx && cloud && params && fb, "resolve: null argument");
+    const int mode = params->mode;
+    const bool hqs = mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16;
+    PCR_CHECK(hqs == (fb->kind == PCR_FB_HQS), "resolve: mode %d does not match the framebuffer kind", mode);
+    PCR_CHECK(hqs || mode == PCR_MODE_PLAIN16 || cloud->d_rgba, "resolve: cloud has no colour plane");
+    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+    int rc = launch_resolve(ctx->stream, cloud, mode, params->background, fb);
+    if (rc) return rc;
+    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+    return PCR_OK;
+}
+
+int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out) {
+    PCR_CHECK(ctx && out, "stats: null argument");
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    FrameState st;
+    PCR_HIP(hipMemcpy(&st, ctx->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof(*out));
+    out->batches_total = ctx->last_nb;
+    out->batches_culled = st.culled;
+    for (int i = 0; i < 5; i++) out->batches_per_level[i] = st.level_count[i];
+    out->points_drawn = st.points_drawn;
+    out->bytes_algorithmic = st.plane_bytes + 64ull * ctx->last_nb;
+    if (ctx->timed) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) out->ms_clear = ms;
+        if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) out->ms_cull = ms;
+        if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) out->ms_raster = ms;
+        if (hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]) == hipSuccess) out->ms_resolve = ms;
+        (void)hipGetLastError();
+    }
+    ctx->last = *out;
+    return PCR_OK;
+}
+
+}  // extern "C"

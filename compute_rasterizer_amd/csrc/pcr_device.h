@@ -1,0 +1,141 @@
+// pcr_device.h — device-side numerics of the rasterizer (gfx950).
+//
+// Fixed fp32 operation order, compiled with -ffp-contract=off and correctly
+// rounded division/sqrt, so that every result is a pure function of its
+// inputs:
+//   mat4*(x,y,z,1): r_i = fma(M[8+i], z, fma(M[4+i], y, fma(M[i], x, M[12+i])))
+//   mat4*(x,y,z,w): r_i = fma(M[12+i], w, fma(M[8+i], z, fma(M[4+i], y, M[i]*x)))
+//   length(v)     : sqrt(fma(z,z, fma(y,y, x*x)))
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/pcr.h"
+
+namespace pcr {
+
+constexpr float kInvSteps30 = 9.3132257461547852e-10f;  // 2^-30
+constexpr float kInvSteps10 = 9.765625e-04f;            // 2^-10
+constexpr uint8_t kCulled = 255;
+
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Frustum test of an AABB against the six Gribb-Hartmann planes of `m`
+// (compute_loop_las/render.cs:134-197): p-vertex per plane, cull if d < 0.
+__device__ inline bool frustum_visible(const float* __restrict__ m, const float mn[3], const float mx[3]) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int axis = i >> 1;
+        const float sgn = (i == 0 || i == 3 || i == 4) ? -1.0f : 1.0f;
+        float p[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float a = m[4 * k + 3], b = m[4 * k + axis];
+            p[k] = sgn < 0.0f ? a - b : a + b;
+        }
+        const float len = __builtin_sqrtf(fma_(p[2], p[2], fma_(p[1], p[1], p[0] * p[0])));
+        const float nx = p[0] / len, ny = p[1] / len, nz = p[2] / len, c = p[3] / len;
+        const float vx = nx > 0.0f ? mx[0] : mn[0];
+        const float vy = ny > 0.0f ? mx[1] : mn[1];
+        const float vz = nz > 0.0f ? mx[2] : mn[2];
+        const float d = fma_(nz, vz, fma_(ny, vy, nx * vx)) + c;
+        if (d < 0.0f) return false;
+    }
+    return true;
+}
+
+// Screen-space length (pixels) of a view-space x offset `len` placed at the
+// AABB centre (compute_loop_las/render.cs:199-215).
+__device__ inline float projected_px(const float mn[3], const float mx[3], float len,
+                                     const float* __restrict__ vw, const float* __restrict__ pr,
+                                     int w, int h) {
+    const float cx = (mn[0] + mx[0]) * 0.5f, cy = (mn[1] + mx[1]) * 0.5f, cz = (mn[2] + mx[2]) * 0.5f;
+    float vc[4], pc[4], pe[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) vc[i] = fma_(vw[8 + i], cz, fma_(vw[4 + i], cy, fma_(vw[i], cx, vw[12 + i])));
+    const float vex = vc[0] + len;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        pc[i] = fma_(pr[12 + i], vc[3], fma_(pr[8 + i], vc[2], fma_(pr[4 + i], vc[1], pr[i] * vc[0])));
+        pe[i] = fma_(pr[12 + i], vc[3], fma_(pr[8 + i], vc[2], fma_(pr[4 + i], vc[1], pr[i] * vex)));
+    }
+    const float pcx = pc[0] / pc[3], pcy = pc[1] / pc[3], pex = pe[0] / pe[3], pey = pe[1] / pe[3];
+    const float fw = (float)w, fh = (float)h;
+    const float scx = (fw * (pcx + 1.0f)) * 0.5f, scy = (fh * (pcy + 1.0f)) * 0.5f;
+    const float sex = (fw * (pex + 1.0f)) * 0.5f, sey = (fh * (pey + 1.0f)) * 0.5f;
+    const float dx = sex - scx, dy = sey - scy;
+    return __builtin_sqrtf(fma_(dy, dy, dx * dx));
+}
+
+// Precision level 0..4 from the projected AABB diagonal
+// (compute_loop_las/render.cs:217-230; first threshold 80 for LOD nodes,
+// compute_loop_nodes/render.cs:199).
+__device__ inline int precision_level(const float mn[3], const float mx[3], const pcr_view& v, float first) {
+    const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    const float radius = __builtin_sqrtf(fma_(dz, dz, fma_(dy, dy, dx * dx)));
+    const float px = projected_px(mn, mx, radius, v.view_world, v.proj, v.width, v.height);
+    if (px < first) return 4;
+    if (px < 200.0f) return 3;
+    if (px < 500.0f) return 2;
+    if (px < 10000.0f) return 1;
+    return 0;
+}
+
+// Number of 4-byte position planes a level reads (levels 2-4 read only the
+// 10-bit plane).
+__host__ __device__ inline int planes_for_level(int level) { return level == 0 ? 3 : (level == 1 ? 2 : 1); }
+
+// Per-batch decode constants: the scale of one integer step and the AABB min.
+struct Decode {
+    float scale[3];
+    float mn[3];
+};
+__device__ inline Decode make_decode(int level, const float mn[3], const float mx[3]) {
+    Decode d;
+    const float f = level <= 1 ? kInvSteps30 : kInvSteps10;
+#pragma unroll
+    for (int a = 0; a < 3; a++) { d.scale[a] = (mx[a] - mn[a]) * f; d.mn[a] = mn[a]; }
+    return d;
+}
+// Decode one point from its plane words (las2/render.cs:340-458).
+template <int LEVEL_CLASS>  // 0: 30 bit, 1: 20 bit, 2: 10 bit
+__device__ __forceinline__ void decode(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi, float p[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const uint32_t l = (lo >> (10 * a)) & 1023u;
+        uint32_t X;
+        if (LEVEL_CLASS == 0) X = (l << 20) | (((med >> (10 * a)) & 1023u) << 10) | ((hi >> (10 * a)) & 1023u);
+        else if (LEVEL_CLASS == 1) X = (l << 20) | (((med >> (10 * a)) & 1023u) << 10);
+        else X = l;
+        p[a] = fma_((float)X, d.scale[a], d.mn[a]);
+    }
+}
+
+// Project a point with `t` (column-major), returning the pixel and the depth
+// bits (clip w = view-space depth) when it lands inside the image
+// (compute_loop_las/render.cs:400-409; truncated pixel == width/height and
+// NaN positions are rejected).
+__device__ __forceinline__ bool project(const float* __restrict__ t, float x, float y, float z, int w, int h,
+                                        uint32_t& pixel, uint32_t& depth_bits) {
+    const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
+    const float py = fma_(t[9], z, fma_(t[5], y, fma_(t[1], x, t[13])));
+    const float pw = fma_(t[11], z, fma_(t[7], y, fma_(t[3], x, t[15])));
+    const float nx = px / pw, ny = py / pw;
+    if (!(pw > 0.0f && nx >= -1.0f && nx <= 1.0f && ny >= -1.0f && ny <= 1.0f)) return false;
+    const float sx = fma_(nx, 0.5f, 0.5f) * (float)w;
+    const float sy = fma_(ny, 0.5f, 0.5f) * (float)h;
+    const int ix = (int)sx, iy = (int)sy;
+    if (ix >= w || iy >= h) return false;
+    pixel = (uint32_t)ix + (uint32_t)iy * (uint32_t)w;
+    depth_bits = __float_as_uint(pw);
+    return true;
+}
+
+// 64-bit fetch-min with the benign early-out compare of render.cs:423-426.
+// The plain load can only be stale-high within a frame (values only
+// decrease), so skipping when key >= old never loses a minimum.
+__device__ __forceinline__ void fb_min(unsigned long long* fb, uint32_t pixel, unsigned long long key) {
+    const unsigned long long old = fb[pixel];
+    if (key < old) atomicMin(fb + pixel, key);
+}
+
+}  // namespace pcr

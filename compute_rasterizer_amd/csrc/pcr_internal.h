@@ -1,0 +1,104 @@
+// pcr_internal.h — library-internal host structures and launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/pcr.h"
+
+namespace pcr {
+
+// Per-frame counters in device memory, zeroed by a kernel at frame start
+// (a kernel rather than a memset so that the frame can be graph-captured).
+struct FrameState {
+    uint32_t list_count;        // batches appended by the select stage
+    uint32_t queue_head;        // work items taken by the raster stage
+    uint32_t level_count[5];
+    uint32_t culled;
+    unsigned long long points_drawn;
+    unsigned long long plane_bytes;
+};
+
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define PCR_HIP(expr)                                                         \
+    do {                                                                      \
+        hipError_t _e = (expr);                                               \
+        if (_e != hipSuccess) return ::pcr::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+#define PCR_CHECK(cond, ...)                                                  \
+    do {                                                                      \
+        if (!(cond)) { ::pcr::set_error(__VA_ARGS__); return PCR_E_INVALID; } \
+    } while (0)
+
+}  // namespace pcr
+
+struct pcr_ctx {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    pcr::FrameState* d_state = nullptr;   // one FrameState per context
+    hipEvent_t ev[8] = {};                // stage timers
+    bool timed = false;                   // last frame recorded stage events
+    pcr_stats last{};                     // host copy of the last frame's stats
+    uint32_t last_nb = 0;
+};
+
+struct pcr_cloud {
+    pcr_ctx* ctx = nullptr;
+    uint64_t num_points = 0;
+    uint32_t num_batches = 0;
+    uint32_t batch_size = 0;
+    uint64_t plane_len = 0;               // words per plane (padded to a multiple of 4)
+    pcr_batch* d_batches = nullptr;
+    uint32_t* d_lo = nullptr;
+    uint32_t* d_med = nullptr;
+    uint32_t* d_hi = nullptr;
+    uint32_t* d_rgba = nullptr;           // colour plane, same index as the position planes
+    uint4* d_points16 = nullptr;          // x, y, z (float bits), rgba
+    uint64_t points16_len = 0;
+    uint8_t* d_levels = nullptr;          // per-batch level of the last frame (255 = not drawn)
+    uint32_t* d_list = nullptr;           // compacted list of drawn batches
+    int num_levels = 0;                   // octree depth + 1 (LOD clouds)
+    float aabb_min[3] = {0, 0, 0};
+    float aabb_max[3] = {0, 0, 0};
+    uint64_t device_bytes = 0;
+    std::vector<pcr_batch> h_batches;     // host copy of the headers
+};
+
+struct pcr_fb {
+    pcr_ctx* ctx = nullptr;
+    int width = 0, height = 0, kind = 0;
+    unsigned long long* d_keys = nullptr; // KEY64
+    uint32_t* d_depth = nullptr;          // HQS
+    uint32_t* d_acc = nullptr;            // HQS: R,G,B,count per pixel
+    uint32_t* d_rgba = nullptr;           // resolved image
+    uint64_t bytes = 0;
+};
+
+namespace pcr {
+
+// kernels (pcr_kernels.hip)
+int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v);
+int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
+int launch_frame_reset(hipStream_t s, FrameState* st);
+int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pcr_render_params& p,
+                  FrameState* st, float first_threshold, bool lod);
+int launch_raster_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int vec, int block,
+                          FrameState* st, unsigned long long* fb);
+int launch_raster_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb);
+int launch_hqs_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb);
+int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pcr_fb* fb);
+int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, pcr_fb* fb);
+
+// layout (pcr_layout.hip)
+int generate_points(pcr_ctx* ctx, const pcr_gen_cfg* cfg, float* xyz, uint32_t* rgba);
+int build_cloud_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, uint64_t n,
+                       const pcr_build_opts* opts, pcr_cloud** out);
+int cloud_alloc_batched(pcr_ctx* ctx, uint32_t nb, uint64_t npoints, pcr_cloud* c);
+void cloud_free(pcr_cloud* c);
+void encode_batches_host(const float* xyz, uint64_t n, uint32_t bs, pcr_batch* hdr,
+                         uint32_t* lo, uint32_t* med, uint32_t* hi);
+
+}  // namespace pcr

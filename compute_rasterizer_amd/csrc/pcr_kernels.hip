@@ -1,0 +1,422 @@
+// pcr_kernels.hip — per-frame kernels for gfx950.
+//
+// Stage map (reference file:line it replaces):
+//   k_select          cull + precision select + LOD cut, one thread per batch
+//                     (compute_loop_las/render.cs:245-254, compute_loop_nodes/render.cs:267-275)
+//   k_raster_batched  decode + project + 64-bit atomicMin, one workgroup per batch
+//                     (compute_loop_las/render.cs:296-428, compute_loop_las2/render.cs:319-469)
+//   k_raster_points16 16 B/pt points, colour payload (compute_2021_earlyz/render.cs:35-63)
+//   k_hqs_*           two-pass depth + accumulate (compute_loop_las_hqs/depth.cs, color.cs)
+//   k_resolve_*       to RGBA8 (compute_loop_las/resolve.cs:100-146, hqs/resolve.cs:62-129,
+//                     compute_2021_earlyz/resolve.cs:41-78)
+#include "pcr_device.h"
+#include "pcr_internal.h"
+
+namespace pcr {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ u32x4 ld_stream(const u32x4* p) { return __builtin_nontemporal_load(p); }
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, off);
+        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
+
+// ------------------------------------------------------------------ fills
+__global__ void k_fill64(unsigned long long* __restrict__ p, uint64_t n, unsigned long long v) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+__global__ void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+__global__ void k_frame_reset(FrameState* st) {
+    if (threadIdx.x == 0) {
+        st->list_count = 0; st->queue_head = 0; st->culled = 0;
+        for (int i = 0; i < 5; i++) st->level_count[i] = 0;
+        st->points_drawn = 0; st->plane_bytes = 0;
+    }
+}
+
+static int grid_for(uint64_t n, int block) {
+    uint64_t g = (n + block - 1) / block;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v) {
+    k_fill64<<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v) {
+    k_fill32<<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+int launch_frame_reset(hipStream_t s, FrameState* st) {
+    k_frame_reset<<<1, 64, 0, s>>>(st);
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+// ----------------------------------------------------------------- select
+// One thread per batch/node: frustum cull, LOD cut, precision level. Writes
+// levels[i] (255 = not drawn), appends drawn batches to `list` with one
+// atomic per wave, and accumulates per-frame statistics.
+template <bool LOD>
+__global__ __launch_bounds__(256) void k_select(const pcr_batch* __restrict__ b, uint32_t nb, pcr_view v,
+                                                int culling, int rule, float lod_px, float first,
+                                                uint8_t* __restrict__ levels, uint32_t* __restrict__ list,
+                                                FrameState* __restrict__ st) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int level = kCulled;
+    uint32_t np = 0;
+    if (i < nb) {
+        const pcr_batch h = b[i];
+        bool vis = !culling || frustum_visible(v.transform_frustum, h.min, h.max);
+        if (LOD && vis) {
+            if (rule == PCR_LOD_SIZE80) {
+                vis = precision_level(h.min, h.max, v, 80.0f) < 4;
+            } else {
+                int32_t p = h.parent;
+                while (vis && p >= 0) {
+                    const pcr_batch a = b[p];
+                    vis = (!culling || frustum_visible(v.transform_frustum, a.min, a.max)) &&
+                          projected_px(a.min, a.max, a.spacing, v.view_world, v.proj, v.width, v.height) > lod_px;
+                    p = a.parent;
+                }
+            }
+        }
+        if (vis) { level = precision_level(h.min, h.max, v, first); np = h.num_points; }
+        levels[i] = (uint8_t)level;
+    }
+    const bool keep = level != kCulled;
+    const unsigned long long m = __ballot(keep);
+    const uint32_t lane = __lane_id();
+    if (m) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&st->list_count, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader);
+        if (keep) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    }
+    const unsigned long long culled = __ballot(i < nb && !keep);
+    uint32_t cnt[5];
+#pragma unroll
+    for (int L = 0; L < 5; L++) cnt[L] = (uint32_t)__popcll(__ballot(level == L));
+    const unsigned long long pts = wave_sum_u64(np);
+    const unsigned long long bytes = wave_sum_u64(keep ? (unsigned long long)np * 4ull * planes_for_level(level) : 0ull);
+    if (lane == 0) {
+        if (culled) atomicAdd(&st->culled, (uint32_t)__popcll(culled));
+#pragma unroll
+        for (int L = 0; L < 5; L++) if (cnt[L]) atomicAdd(&st->level_count[L], cnt[L]);
+        if (pts) atomicAdd(&st->points_drawn, pts);
+        if (bytes) atomicAdd(&st->plane_bytes, bytes);
+    }
+}
+
+int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pcr_render_params& p,
+                  FrameState* st, float first, bool lod) {
+    const int grid = (int)((c->num_batches + 255) / 256);
+    if (grid == 0) return PCR_OK;
+    if (lod)
+        k_select<true><<<grid, 256, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling, p.lod_rule,
+                                            p.lod_px, first, c->d_levels, c->d_list, st);
+    else
+        k_select<false><<<grid, 256, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling, p.lod_rule,
+                                             p.lod_px, first, c->d_levels, c->d_list, st);
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+// ----------------------------------------------------------------- raster
+template <int LC>
+__device__ __forceinline__ void raster_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi,
+                                             uint32_t local, uint32_t np, uint32_t first, const pcr_view& v,
+                                             unsigned long long* __restrict__ fb) {
+    if (local >= np) return;
+    float p[3];
+    decode<LC>(d, lo, med, hi, p);
+    uint32_t pix, depth;
+    if (!project(v.transform, p[0], p[1], p[2], v.width, v.height, pix, depth)) return;
+    fb_min(fb, pix, ((unsigned long long)depth << 32) | (first + local));
+}
+
+// Four consecutive points per thread per iteration, 128-bit streaming loads of
+// only the planes the level needs (compute_loop_las2/render.cs:319-469).
+template <int BLOCK, int LC>
+__device__ __forceinline__ void raster_vec4(const Decode& d, const uint32_t* __restrict__ lo,
+                                            const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                            uint32_t first, uint32_t np, const pcr_view& v,
+                                            unsigned long long* __restrict__ fb) {
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + first);
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + first);
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + first);
+    const uint32_t nq = (np + 3u) >> 2;
+    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
+        const u32x4 a = ld_stream(lo4 + q);
+        const u32x4 m = LC <= 1 ? ld_stream(med4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 c = LC == 0 ? ld_stream(hi4 + q) : u32x4{0, 0, 0, 0};
+        const uint32_t base = q * 4u;
+        raster_point<LC>(d, a.x, m.x, c.x, base + 0, np, first, v, fb);
+        raster_point<LC>(d, a.y, m.y, c.y, base + 1, np, first, v, fb);
+        raster_point<LC>(d, a.z, m.z, c.z, base + 2, np, first, v, fb);
+        raster_point<LC>(d, a.w, m.w, c.w, base + 3, np, first, v, fb);
+    }
+}
+
+// One point per thread per iteration with stride BLOCK
+// (compute_loop_las/render.cs:296-318, RENDER_STRIDE_128).
+template <int BLOCK, int LC>
+__device__ __forceinline__ void raster_strided(const Decode& d, const uint32_t* __restrict__ lo,
+                                               const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                               uint32_t first, uint32_t np, const pcr_view& v,
+                                               unsigned long long* __restrict__ fb) {
+    for (uint32_t i = threadIdx.x; i < np; i += BLOCK) {
+        const uint32_t idx = first + i;
+        raster_point<LC>(d, lo[idx], LC <= 1 ? med[idx] : 0u, LC == 0 ? hi[idx] : 0u, i, np, first, v, fb);
+    }
+}
+
+template <int BLOCK, int VEC>
+__global__ __launch_bounds__(BLOCK) void k_raster_batched(const pcr_batch* __restrict__ b,
+                                                          const uint8_t* __restrict__ levels,
+                                                          const uint32_t* __restrict__ lo,
+                                                          const uint32_t* __restrict__ med,
+                                                          const uint32_t* __restrict__ hi, pcr_view v,
+                                                          unsigned long long* __restrict__ fb) {
+    const uint32_t bi = blockIdx.x;
+    const int level = levels[bi];
+    if (level == kCulled) return;
+    const pcr_batch h = b[bi];
+    const Decode d = make_decode(level, h.min, h.max);
+    if (VEC == 4) {
+        if (level == 0) raster_vec4<BLOCK, 0>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
+        else if (level == 1) raster_vec4<BLOCK, 1>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
+        else raster_vec4<BLOCK, 2>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
+    } else {
+        if (level == 0) raster_strided<BLOCK, 0>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
+        else if (level == 1) raster_strided<BLOCK, 1>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
+        else raster_strided<BLOCK, 2>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
+    }
+}
+
+template <int BLOCK, int VEC>
+static void raster_batched_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb) {
+    k_raster_batched<BLOCK, VEC><<<c->num_batches, BLOCK, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
+                                                                  c->d_hi, v, fb);
+}
+
+int launch_raster_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int vec, int block,
+                          FrameState* st, unsigned long long* fb) {
+    (void)st;
+    if (c->num_batches == 0) return PCR_OK;
+    if (vec == 4) {
+        switch (block) {
+            case 64: raster_batched_t<64, 4>(s, c, v, fb); break;
+            case 128: raster_batched_t<128, 4>(s, c, v, fb); break;
+            case 512: raster_batched_t<512, 4>(s, c, v, fb); break;
+            default: raster_batched_t<256, 4>(s, c, v, fb); break;
+        }
+    } else {
+        switch (block) {
+            case 64: raster_batched_t<64, 1>(s, c, v, fb); break;
+            case 256: raster_batched_t<256, 1>(s, c, v, fb); break;
+            case 512: raster_batched_t<512, 1>(s, c, v, fb); break;
+            default: raster_batched_t<128, 1>(s, c, v, fb); break;
+        }
+    }
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+// 16 B/pt points (x, y, z, rgba), colour payload.
+__global__ __launch_bounds__(256) void k_raster_points16(const u32x4* __restrict__ pts, uint64_t n, pcr_view v,
+                                                         unsigned long long* __restrict__ fb) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32x4 p = ld_stream(pts + i);
+        uint32_t pix, depth;
+        if (!project(v.transform, __uint_as_float(p.x), __uint_as_float(p.y), __uint_as_float(p.z), v.width,
+                     v.height, pix, depth))
+            continue;
+        fb_min(fb, pix, ((unsigned long long)depth << 32) | p.w);
+    }
+}
+
+int launch_raster_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb) {
+    if (c->points16_len == 0) return PCR_OK;
+    uint64_t g = (c->points16_len + 255) / 256;
+    if (g > (uint64_t)c->ctx->cu_count * 64) g = (uint64_t)c->ctx->cu_count * 64;
+    k_raster_points16<<<(int)g, 256, 0, s>>>(reinterpret_cast<const u32x4*>(c->d_points16), c->points16_len, v, fb);
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+// -------------------------------------------------------------------- hqs
+// Pass 1 (depth.cs:371-379): 32-bit atomicMin of the depth bits.
+// Pass 2 (color.cs:398-437): points within 1.01x of the pixel's minimum depth
+// add R, G, B and 1 to four uint32 accumulators.
+__device__ __forceinline__ void hqs_depth(uint32_t* __restrict__ depth, uint32_t pix, uint32_t d) {
+    if (d < depth[pix]) atomicMin(depth + pix, d);
+}
+__device__ __forceinline__ void hqs_accum(const uint32_t* __restrict__ depth, uint32_t* __restrict__ acc,
+                                          uint32_t pix, uint32_t d, uint32_t color) {
+    const float bd = __uint_as_float(depth[pix]);
+    if (!(__uint_as_float(d) <= bd * 1.01f)) return;
+    uint32_t* a = acc + 4ull * pix;
+    atomicAdd(a + 0, color & 255u);
+    atomicAdd(a + 1, (color >> 8) & 255u);
+    atomicAdd(a + 2, (color >> 16) & 255u);
+    atomicAdd(a + 3, 1u);
+}
+
+template <int PASS, int LC>
+__device__ __forceinline__ void hqs_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi, uint32_t color,
+                                          uint32_t local, uint32_t np, const pcr_view& v,
+                                          uint32_t* __restrict__ depth, uint32_t* __restrict__ acc) {
+    if (local >= np) return;
+    float p[3];
+    decode<LC>(d, lo, med, hi, p);
+    uint32_t pix, dep;
+    if (!project(v.transform, p[0], p[1], p[2], v.width, v.height, pix, dep)) return;
+    if (PASS == 0) hqs_depth(depth, pix, dep);
+    else hqs_accum(depth, acc, pix, dep, color);
+}
+
+template <int BLOCK, int PASS, int LC>
+__device__ __forceinline__ void hqs_vec4(const Decode& d, const uint32_t* __restrict__ lo,
+                                         const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                         const uint32_t* __restrict__ rgba, uint32_t first, uint32_t np,
+                                         const pcr_view& v, uint32_t* __restrict__ depth, uint32_t* __restrict__ acc) {
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + first);
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + first);
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + first);
+    const u32x4* c4 = reinterpret_cast<const u32x4*>(rgba + first);
+    const uint32_t nq = (np + 3u) >> 2;
+    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
+        const u32x4 a = ld_stream(lo4 + q);
+        const u32x4 m = LC <= 1 ? ld_stream(med4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 c = LC == 0 ? ld_stream(hi4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 col = PASS == 1 ? ld_stream(c4 + q) : u32x4{0, 0, 0, 0};
+        const uint32_t base = q * 4u;
+        hqs_point<PASS, LC>(d, a.x, m.x, c.x, col.x, base + 0, np, v, depth, acc);
+        hqs_point<PASS, LC>(d, a.y, m.y, c.y, col.y, base + 1, np, v, depth, acc);
+        hqs_point<PASS, LC>(d, a.z, m.z, c.z, col.z, base + 2, np, v, depth, acc);
+        hqs_point<PASS, LC>(d, a.w, m.w, c.w, col.w, base + 3, np, v, depth, acc);
+    }
+}
+
+template <int BLOCK, int PASS>
+__global__ __launch_bounds__(BLOCK) void k_hqs_batched(const pcr_batch* __restrict__ b,
+                                                       const uint8_t* __restrict__ levels,
+                                                       const uint32_t* __restrict__ lo,
+                                                       const uint32_t* __restrict__ med,
+                                                       const uint32_t* __restrict__ hi,
+                                                       const uint32_t* __restrict__ rgba, pcr_view v,
+                                                       uint32_t* __restrict__ depth, uint32_t* __restrict__ acc) {
+    const uint32_t bi = blockIdx.x;
+    const int level = levels[bi];
+    if (level == kCulled) return;
+    const pcr_batch h = b[bi];
+    const Decode d = make_decode(level, h.min, h.max);
+    if (level == 0) hqs_vec4<BLOCK, PASS, 0>(d, lo, med, hi, rgba, h.first_point, h.num_points, v, depth, acc);
+    else if (level == 1) hqs_vec4<BLOCK, PASS, 1>(d, lo, med, hi, rgba, h.first_point, h.num_points, v, depth, acc);
+    else hqs_vec4<BLOCK, PASS, 2>(d, lo, med, hi, rgba, h.first_point, h.num_points, v, depth, acc);
+}
+
+int launch_hqs_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb) {
+    (void)st;
+    if (c->num_batches == 0) return PCR_OK;
+    k_hqs_batched<256, 0><<<c->num_batches, 256, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med, c->d_hi,
+                                                         c->d_rgba, v, fb->d_depth, fb->d_acc);
+    PCR_HIP(hipGetLastError());
+    k_hqs_batched<256, 1><<<c->num_batches, 256, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med, c->d_hi,
+                                                         c->d_rgba, v, fb->d_depth, fb->d_acc);
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+template <int PASS>
+__global__ __launch_bounds__(256) void k_hqs_points16(const u32x4* __restrict__ pts, uint64_t n, pcr_view v,
+                                                      uint32_t* __restrict__ depth, uint32_t* __restrict__ acc) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const u32x4 p = ld_stream(pts + i);
+        uint32_t pix, dep;
+        if (!project(v.transform, __uint_as_float(p.x), __uint_as_float(p.y), __uint_as_float(p.z), v.width,
+                     v.height, pix, dep))
+            continue;
+        if (PASS == 0) hqs_depth(depth, pix, dep);
+        else hqs_accum(depth, acc, pix, dep, p.w);
+    }
+}
+
+int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pcr_fb* fb) {
+    if (c->points16_len == 0) return PCR_OK;
+    uint64_t g = (c->points16_len + 255) / 256;
+    if (g > (uint64_t)c->ctx->cu_count * 64) g = (uint64_t)c->ctx->cu_count * 64;
+    const u32x4* pts = reinterpret_cast<const u32x4*>(c->d_points16);
+    k_hqs_points16<0><<<(int)g, 256, 0, s>>>(pts, c->points16_len, v, fb->d_depth, fb->d_acc);
+    PCR_HIP(hipGetLastError());
+    k_hqs_points16<1><<<(int)g, 256, 0, s>>>(pts, c->points16_len, v, fb->d_depth, fb->d_acc);
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+// ---------------------------------------------------------------- resolve
+// Index payload: colour gathered from the colour plane (resolve.cs:117-146).
+// An index outside the cloud (a framebuffer filled by another cloud or mode)
+// resolves to the background instead of reading out of bounds.
+__global__ void k_resolve_index(const unsigned long long* __restrict__ fb, uint64_t npix,
+                                const uint32_t* __restrict__ rgba, uint64_t npoints, uint32_t bg,
+                                uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
+        const unsigned long long k = fb[i];
+        const uint32_t id = (uint32_t)k;
+        out[i] = (k == PCR_FB_CLEAR || id >= npoints) ? bg : rgba[id];
+    }
+}
+// Colour payload: the key's low 32 bits are the RGBA (2021 resolve.cs:45-57).
+__global__ void k_resolve_color(const unsigned long long* __restrict__ fb, uint64_t npix, uint32_t bg,
+                                uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
+        const unsigned long long k = fb[i];
+        out[i] = k == PCR_FB_CLEAR ? bg : (uint32_t)k;
+    }
+}
+// hqs/resolve.cs:74-129: integer averages, background where count == 0.
+__global__ void k_resolve_hqs(const u32x4* __restrict__ acc, uint64_t npix, uint32_t bg, uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
+        const u32x4 a = acc[i];
+        out[i] = a.w == 0 ? bg : ((a.x / a.w) | ((a.y / a.w) << 8) | ((a.z / a.w) << 16) | 0xFF000000u);
+    }
+}
+
+int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, pcr_fb* fb) {
+    const uint64_t npix = (uint64_t)fb->width * fb->height;
+    const int g = grid_for(npix, 256);
+    if (mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16) {
+        k_resolve_hqs<<<g, 256, 0, s>>>(reinterpret_cast<const u32x4*>(fb->d_acc), npix, bg, fb->d_rgba);
+    } else if (mode == PCR_MODE_PLAIN16) {
+        k_resolve_color<<<g, 256, 0, s>>>(fb->d_keys, npix, bg, fb->d_rgba);
+    } else {
+        k_resolve_index<<<g, 256, 0, s>>>(fb->d_keys, npix, c->d_rgba, c->num_points, bg, fb->d_rgba);
+    }
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+}  // namespace pcr

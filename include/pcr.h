@@ -1,0 +1,325 @@
+/*
+ * pcr.h — C ABI of the MI355X point-cloud rasterizer (libpcr.so).
+ *
+ * This is the drop-in boundary for the reference's per-frame Method plugin
+ * (include/Method.h:10-24: `update(Renderer*)` / `render(Renderer*)` over a
+ * loaded point buffer into a framebuffer). Every entry point takes plain
+ * pointers and sizes; there are no GL, torch or C++ types in the signatures.
+ *
+ * Conventions
+ *   - Every function returns PCR_OK (0) or a negative PCR_E_* code; the text
+ *     of the last error on the calling thread is in pcr_last_error().
+ *   - Matrices are float[16], column-major (glm layout): m[col*4 + row].
+ *   - Framebuffers are row-major, pixel (x, y) at index x + y*width.
+ *   - Host pointers are named *_host; device pointers returned by the library
+ *     are named *_dev. Work is queued on the context's HIP stream and is
+ *     asynchronous unless the function says it synchronises.
+ *   - Thread safety: one context per host thread, except the pcr_server_*
+ *     family, whose submit/poll calls may come from any thread.
+ */
+#ifndef PCR_H
+#define PCR_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCR_VERSION_MAJOR 1
+#define PCR_VERSION_MINOR 0
+
+/* ---- status codes ------------------------------------------------------ */
+#define PCR_OK               0
+#define PCR_E_INVALID      (-1)  /* bad argument (null, size, alignment, mode) */
+#define PCR_E_NO_DEVICE    (-2)  /* HIP runtime has no usable GPU            */
+#define PCR_E_HIP          (-3)  /* a HIP runtime call failed                 */
+#define PCR_E_OOM          (-4)  /* device allocation failed                 */
+#define PCR_E_BUDGET       (-5)  /* request refused by the framebuffer budget */
+#define PCR_E_NOT_READY    (-6)  /* server ticket still in flight             */
+#define PCR_E_CANCELLED    (-7)  /* server ticket cancelled                   */
+
+/* ---- fixed layout constants (modules/compute/Resources.h:4-6) --------- */
+#define PCR_POINTS_PER_BATCH      10240u   /* 80 points x 128 threads       */
+#define PCR_FB_CLEAR        0xFFFFFFFFFFFFFFFFull
+#define PCR_DEPTH_CLEAR     0xFFFFFFFFu
+#define PCR_BG_LAS          0x00443322u    /* compute_loop_las/resolve.cs:141 */
+#define PCR_BG_BLACK        0xFF000000u
+
+/*
+ * 64-byte batch / node header.
+ * Replaces `struct Batch` of compute_loop_las/render.cs:22-40 (written by
+ * LasLoaderSparse.cpp:110-121) and the node variant of
+ * compute_loop_nodes/render.cs:26-43 (`pointOffset`@32, `level`@36).
+ * min/max are the float AABB of the batch's points in cloud coordinates
+ * (relative to the cloud origin). first_point must be a multiple of 4 so that
+ * the position planes can be read with 128-bit loads.
+ */
+typedef struct pcr_batch {
+    int32_t  state;        /* @0  unused, 0                                   */
+    float    min[3];       /* @4                                              */
+    float    max[3];       /* @16                                             */
+    uint32_t num_points;   /* @28                                             */
+    uint32_t first_point;  /* @32 index of the batch's first point            */
+    uint32_t level;        /* @36 octree level (LOD nodes), else file index 0 */
+    int32_t  parent;       /* @40 parent node index (LOD nodes), -1 = root    */
+    float    spacing;      /* @44 node point spacing (LOD nodes)              */
+    uint32_t pad[4];       /* @48 .. 64                                       */
+} pcr_batch;
+
+/*
+ * One camera / render target. Replaces the UBO and per-file SSBO that
+ * ComputeLoopLas::render fills (compute_loop_las.h:141-199): `transform` is
+ * float(proj*view*world) computed in double, `transform_frustum` the matrix
+ * the frustum planes are taken from (normally equal to transform),
+ * `view_world` = float(view*world) in double and `proj` the float projection,
+ * both used by the precision select (render.cs:199-231).
+ */
+typedef struct pcr_view {
+    float   transform[16];
+    float   transform_frustum[16];
+    float   view_world[16];
+    float   proj[16];
+    int32_t width;
+    int32_t height;
+    int32_t pad[2];
+} pcr_view;
+
+/* render modes: which reference Method the call stands in for */
+typedef enum pcr_mode {
+    PCR_MODE_PLAIN16  = 0,  /* 16 B/pt points, colour payload (compute_2021_earlyz)         */
+    PCR_MODE_LAS      = 1,  /* batched adaptive precision, index payload (compute_loop_las) */
+    PCR_MODE_LAS2     = 2,  /* same, 4-consecutive-point vector loads  (compute_loop_las2)  */
+    PCR_MODE_HQS      = 3,  /* two-pass depth + accumulate            (compute_loop_las_hqs)*/
+    PCR_MODE_NODES    = 4,  /* LOD octree nodes                       (compute_loop_nodes)  */
+    PCR_MODE_HQS16    = 5   /* hqs over the 16 B/pt layout                                 */
+} pcr_mode;
+
+/* LOD criteria for PCR_MODE_NODES */
+typedef enum pcr_lod_rule {
+    PCR_LOD_SPACING   = 0,  /* refine a node while its projected spacing > lod_px      */
+    PCR_LOD_SIZE80    = 1   /* reference: skip nodes whose projected size < 80 px      */
+} pcr_lod_rule;
+
+typedef struct pcr_render_params {
+    int32_t  mode;              /* pcr_mode                                        */
+    int32_t  frustum_culling;   /* uniforms.enableFrustumCulling (render.cs:250)   */
+    int32_t  lod_rule;          /* pcr_lod_rule (PCR_MODE_NODES only)              */
+    float    lod_px;            /* LOD screen-space error threshold in pixels      */
+    uint32_t background;        /* RGBA8 written where nothing was drawn           */
+    int32_t  block_size;        /* raster workgroup size: 0 = default (256)        */
+    int32_t  clear_before;      /* 1: clear the framebuffer before rasterising     */
+    int32_t  time_stages;       /* 1: record HIP events around each stage          */
+} pcr_render_params;
+
+typedef struct pcr_stats {
+    uint64_t batches_total;
+    uint64_t batches_culled;       /* frustum-culled or LOD-skipped               */
+    uint64_t batches_per_level[5]; /* precision level histogram of drawn batches  */
+    uint64_t points_drawn;         /* points in non-culled batches                */
+    uint64_t bytes_algorithmic;    /* headers + planes read by the raster stage   */
+    float    ms_cull;              /* stage times of the last timed frame          */
+    float    ms_raster;
+    float    ms_resolve;
+    float    ms_clear;
+} pcr_stats;
+
+typedef struct pcr_ctx   pcr_ctx;
+typedef struct pcr_cloud pcr_cloud;
+typedef struct pcr_fb    pcr_fb;
+
+/* ---- context ---------------------------------------------------------- */
+const char* pcr_last_error(void);
+int pcr_version(void);                         /* major*100 + minor */
+int pcr_device_count(int* count);
+int pcr_create(int device, pcr_ctx** out);
+int pcr_destroy(pcr_ctx* ctx);
+int pcr_synchronize(pcr_ctx* ctx);
+void* pcr_stream(pcr_ctx* ctx);                /* the context's hipStream_t */
+
+/* ---- host-side helpers (no device work) ------------------------------- */
+/* Build a pcr_view for a look-at perspective camera; all matrix algebra in
+ * double, glm lookAtRH / perspectiveRH_NO conventions (Camera.h:34-38,
+ * matrix_transform.inl:258-274), world = translate(origin) as in
+ * compute_loop_las.h:170-176. fovy in degrees. */
+int pcr_view_look_at(const double eye[3], const double target[3], const double up[3],
+                     double fovy_deg, double near_plane, double far_plane,
+                     const double origin[3], int width, int height, pcr_view* out);
+
+/* Encode points into the adaptive-precision batch layout on the host
+ * (LasLoaderSparse.cpp:37-212). Points are taken in the given order; batches
+ * of `batch_size` points; planes lo/med/hi hold bits 29..20 / 19..10 / 9..0. */
+int pcr_encode_batches_host(const float* xyz_host, uint64_t n, uint32_t batch_size,
+                            pcr_batch* batches_host, uint32_t* lo_host,
+                            uint32_t* med_host, uint32_t* hi_host);
+
+/* ---- clouds ------------------------------------------------------------ */
+/* 16 B/pt layout: xyz float + RGBA8 (LasLoaderStandard.h:152-178). */
+int pcr_cloud_upload_points16(pcr_ctx* ctx, const float* xyz_host, const uint32_t* rgba_host,
+                              uint64_t n, pcr_cloud** out);
+/* Adaptive-precision layout already encoded by the caller (the output of the
+ * reference's LasLoaderSparse, modules/compute/LasLoaderSparse.h:46-99). */
+int pcr_cloud_upload_batched(pcr_ctx* ctx, const pcr_batch* batches_host, uint32_t nbatches,
+                             const uint32_t* lo_host, const uint32_t* med_host,
+                             const uint32_t* hi_host, const uint32_t* rgba_host,
+                             uint64_t npoints, pcr_cloud** out);
+
+/* Synthetic scene generators, evaluated on the device from a counter-based
+ * PRNG so that every point depends only on (seed, index). */
+typedef enum pcr_scene {
+    PCR_SCENE_CUBE    = 0,   /* xyz ~ U[0,1)^3, rgba = uint8(255*xyz), A=255       */
+    PCR_SCENE_TERRAIN = 1,   /* fBm heightfield terrain                            */
+    PCR_SCENE_CITY    = 2    /* box buildings on fBm terrain                       */
+} pcr_scene;
+
+typedef struct pcr_gen_cfg {
+    int32_t  scene;          /* pcr_scene                                        */
+    int32_t  num_buildings;  /* CITY                                             */
+    uint64_t seed;
+    uint64_t num_points;
+    float    extent;         /* TERRAIN/CITY: side of the square area (m)        */
+    float    amplitude;      /* TERRAIN/CITY: fBm amplitude (m)                  */
+    float    jitter;         /* TERRAIN: +-jitter (m)                            */
+    float    terrain_frac;   /* CITY: fraction of points on the terrain          */
+} pcr_gen_cfg;
+
+/* Generate points on the device: xyz float[3n] + rgba uint32[n], unsorted. */
+int pcr_generate_points(pcr_ctx* ctx, const pcr_gen_cfg* cfg, float* xyz_dev, uint32_t* rgba_dev);
+
+/* Build options for the device-side layout builder. */
+typedef struct pcr_build_opts {
+    uint32_t batch_size;     /* points per batch, multiple of 4; 0 = 10240       */
+    int32_t  morton_sort;    /* 1: sort by 63-bit Morton code before batching    */
+    int32_t  keep_points16;  /* 1: also keep the sorted 16 B/pt copy             */
+} pcr_build_opts;
+
+/* Morton-sort, batch and encode points that are already on the device. */
+int pcr_cloud_build_device(pcr_ctx* ctx, const float* xyz_dev, const uint32_t* rgba_dev,
+                           uint64_t n, const pcr_build_opts* opts, pcr_cloud** out);
+/* Generate + build in one call (frees the temporaries). */
+int pcr_cloud_generate(pcr_ctx* ctx, const pcr_gen_cfg* cfg, const pcr_build_opts* opts,
+                       pcr_cloud** out);
+
+typedef struct pcr_cloud_info {
+    uint64_t num_points;
+    uint32_t num_batches;
+    uint32_t batch_size;
+    int32_t  has_points16;
+    int32_t  has_batches;
+    int32_t  num_levels;     /* LOD octree depth (0 = no octree)                 */
+    float    aabb_min[3];
+    float    aabb_max[3];
+    uint64_t device_bytes;
+} pcr_cloud_info;
+
+int pcr_cloud_info_get(const pcr_cloud* cloud, pcr_cloud_info* out);
+/* Copy the layout back to host arrays (any pointer may be NULL to skip). */
+int pcr_cloud_download(pcr_cloud* cloud, pcr_batch* batches_host, uint32_t* lo_host,
+                       uint32_t* med_host, uint32_t* hi_host, uint32_t* rgba_host,
+                       float* xyz16_host, uint32_t* rgba16_host);
+int pcr_cloud_destroy(pcr_cloud* cloud);
+
+/* ---- LOD octree (Potree-style nodes) ----------------------------------- */
+typedef struct pcr_octree_opts {
+    uint64_t seed;            /* priority PRNG seed for the subsampling          */
+    float    root_spacing_div;/* root spacing = cube side / this (128)           */
+    uint32_t max_node_points; /* node chunk size limit (20000)                   */
+    int32_t  max_depth;       /* deepest level                                    */
+} pcr_octree_opts;
+
+/* Build a Potree-style octree from points on the host (PotreeData.h:188-317
+ * stands in as the node layout). Nodes become pcr_batch headers with
+ * level/parent/spacing set; node chunks are stored in node order. */
+int pcr_cloud_build_octree_host(pcr_ctx* ctx, const float* xyz_host, const uint32_t* rgba_host,
+                                uint64_t n, const pcr_octree_opts* opts, pcr_cloud** out);
+/* Same, for points already on the device (sorting and level assignment on device). */
+int pcr_cloud_build_octree_device(pcr_ctx* ctx, const float* xyz_dev, const uint32_t* rgba_dev,
+                                  uint64_t n, const pcr_octree_opts* opts, pcr_cloud** out);
+/* The selected node set of the last PCR_MODE_NODES render: flags[i] = 1 if drawn. */
+int pcr_lod_selection(pcr_ctx* ctx, pcr_cloud* cloud, uint8_t* flags_host, uint32_t n);
+
+/* ---- framebuffers ------------------------------------------------------ */
+typedef enum pcr_fb_kind {
+    PCR_FB_KEY64 = 0,   /* uint64 depth|payload keys            */
+    PCR_FB_HQS   = 1    /* uint32 depth + 4 x uint32 R,G,B,count */
+} pcr_fb_kind;
+
+int pcr_fb_create(pcr_ctx* ctx, int width, int height, int kind, pcr_fb** out);
+int pcr_fb_destroy(pcr_fb* fb);
+int pcr_fb_clear(pcr_ctx* ctx, pcr_fb* fb);
+/* Read back (synchronises): keys uint64[w*h], or depth uint32[w*h] + acc uint32[4*w*h]. */
+int pcr_fb_readback(pcr_ctx* ctx, pcr_fb* fb, uint64_t* keys_host);
+int pcr_fb_readback_hqs(pcr_ctx* ctx, pcr_fb* fb, uint32_t* depth_host, uint32_t* acc_host);
+int pcr_fb_readback_rgba(pcr_ctx* ctx, pcr_fb* fb, uint32_t* rgba_host);
+void* pcr_fb_device_keys(pcr_fb* fb);
+void* pcr_fb_device_rgba(pcr_fb* fb);
+
+/* ---- per-frame render (Method::render, compute_loop_las.h:117-304) ------ */
+/* Rasterise `cloud` for one view into `fb` (asynchronous). */
+int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view,
+               const pcr_render_params* params, pcr_fb* fb);
+/* Resolve fb to RGBA8 in the fb's own image (compute_loop_las/resolve.cs:66-195,
+ * hqs/resolve.cs:62-129, compute_2021_earlyz/resolve.cs:41-78). */
+int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params, pcr_fb* fb);
+int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out);   /* synchronises */
+
+/* ---- batched multi-view (persistent launch, hipGraph) ------------------- */
+typedef struct pcr_tune {
+    int32_t block_size;      /* threads per workgroup: 64,128,256,512            */
+    int32_t points_per_thread;/* 0 = derived from batch size                      */
+    int32_t persistent_groups;/* 0 = derived from the device                      */
+} pcr_tune;
+
+typedef struct pcr_multiview pcr_multiview;
+/* Prepare one launch that renders `nviews` views of `cloud` into `fbs`, walking
+ * a (view, batch) work queue; the whole frame (clear, cull, raster, resolve) is
+ * captured in a hipGraph when use_graph != 0. */
+int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nviews,
+                         pcr_fb* const* fbs, const pcr_render_params* params,
+                         const pcr_tune* tune, int use_graph, pcr_multiview** out);
+int pcr_multiview_set_views(pcr_multiview* mv, const pcr_view* views, int nviews);
+int pcr_multiview_launch(pcr_multiview* mv);
+int pcr_multiview_destroy(pcr_multiview* mv);
+/* Sweep block size / points per thread for this cloud and views; writes the
+ * fastest configuration and its time. */
+int pcr_autotune(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nviews,
+                 pcr_fb* const* fbs, const pcr_render_params* params, int iters,
+                 pcr_tune* best, float* best_ms);
+
+/* ---- serving: continuous batching over device-resident clouds ----------- */
+typedef struct pcr_server pcr_server;
+typedef struct pcr_request {
+    int32_t  cloud_id;       /* index returned by pcr_server_add_cloud          */
+    int32_t  mode;           /* PCR_MODE_LAS or PCR_MODE_NODES                   */
+    float    lod_px;
+    int32_t  pad;
+    pcr_view view;
+} pcr_request;
+
+typedef struct pcr_server_stats {
+    uint64_t completed;
+    uint64_t refused;
+    uint64_t launches;
+    uint64_t points_rendered;   /* sum over completed requests of cloud points */
+    uint64_t fb_bytes_in_use;
+    uint64_t fb_bytes_peak;
+} pcr_server_stats;
+
+int pcr_server_create(pcr_ctx* ctx, uint64_t fb_budget_bytes, int max_batch, int num_streams,
+                      pcr_server** out);
+int pcr_server_add_cloud(pcr_server* srv, pcr_cloud* cloud, int* cloud_id);
+/* Enqueue a request; returns a ticket. Refused with PCR_E_BUDGET only when the
+ * request alone exceeds the budget; otherwise it waits for memory to free up. */
+int pcr_server_submit(pcr_server* srv, const pcr_request* req, uint64_t* ticket);
+/* Non-blocking: PCR_OK with the image copied to rgba_host (w*h uint32), or PCR_E_NOT_READY. */
+int pcr_server_poll(pcr_server* srv, uint64_t ticket, uint32_t* rgba_host, double* latency_ms);
+int pcr_server_wait(pcr_server* srv, uint64_t ticket, uint32_t* rgba_host, double* latency_ms);
+int pcr_server_cancel(pcr_server* srv, uint64_t ticket);
+int pcr_server_stats_get(pcr_server* srv, pcr_server_stats* out);
+int pcr_server_destroy(pcr_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PCR_H */
