@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""bench.py — 1e9-point adaptive-precision frames at 1920x1080 on one MI355X.
+
+Workload: a city-scale cloud of 1e9 points (256 box buildings sampled on
+their faces over a 2 km fBm terrain), generated on the device, Morton-sorted
+into 10,240-point batches with 30-bit coordinates in three 10-bit planes.
+One step = one frame of a 64-view seeded flythrough at 250 m: clear, cull +
+precision select, rasterise (64-bit atomicMin), resolve to RGBA8.
+
+Prints ONE JSON line (metric, value, ms_per_step, p50/p99 frame time,
+roofline of the raster kernel, CPU-oracle baseline on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import compute_rasterizer_amd as pcr  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def flythrough(n, width, height, seed=42, extent=2000.0, altitude=250.0):
+    """Seeded circular flythrough above the city, looking ahead and down."""
+    rng = np.random.default_rng(seed)
+    views = []
+    c = extent / 2
+    for k in range(n):
+        a = 2 * math.pi * k / n + rng.uniform(-0.02, 0.02)
+        r = 650.0 + rng.uniform(-40, 40)
+        eye = (c + r * math.cos(a), c + r * math.sin(a), altitude + rng.uniform(-10, 10))
+        ahead = a + math.pi / 2 + rng.uniform(-0.3, 0.3)
+        tgt = (eye[0] + 400 * math.cos(ahead), eye[1] + 400 * math.sin(ahead), 60.0)
+        views.append(pcr.look_at(eye, tgt, up=(0, 0, 1), fovy=60.0, near=1.0, far=5000.0, width=width,
+                                 height=height))
+    return views
+
+
+def cpu_baseline(ctx, cloud, views, gpu_fb, prm, frames, n_points):
+    """Oracle (multithreaded C) on the host cores for `frames` full frames;
+    the GPU framebuffers of the same views are compared bit for bit."""
+    from oracle import orc
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    lay = cloud.download()
+    times, exact = [], True
+    for j in range(frames):
+        v = views[j]
+        t0 = time.perf_counter()
+        ref = orc.render_batched_mt(lay["batches"], lay["lo"], lay["med"], lay["hi"], v, 1, cores)
+        times.append(time.perf_counter() - t0)
+        ctx.render(cloud, v, prm, gpu_fb)
+        exact = exact and bool(np.array_equal(gpu_fb.keys(), ref))
+    del lay
+    cpu_s = float(np.median(times))
+    return {"value": round(n_points / cpu_s / 1e9, 4), "unit": "Gpoints/s", "cores": cores, "kind": "port",
+            "sample": f"{frames} full 1920x1080 frames of the 1e9-point cloud (median {cpu_s:.2f} s/frame), "
+                      f"oracle/pcr_oracle.c multithreaded fetch-min, framebuffers bit-exact to the GPU: {exact}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=128)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--points", type=int, default=1_000_000_000)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--views", type=int, default=64)
+    ap.add_argument("--mode", default="las2", choices=["las", "las2"])
+    ap.add_argument("--cpu-frames", type=int, default=2, help="0 skips the CPU baseline")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build renders on one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not "
+              f"supported. Multi-GPU sharding is described in DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    ctx = pcr.Context(0)
+    t = time.perf_counter()
+    cfg = pcr.gen_cfg(pcr.SCENE_CITY, args.points, seed=42, num_buildings=256, terrain_frac=0.2)
+    cloud = ctx.generate(cfg)
+    gen_s = time.perf_counter() - t
+    info = cloud.info()
+    print(f"[bench] generated {info.num_points} points in {info.num_batches} batches ({gen_s:.1f} s)",
+          file=sys.stderr, flush=True)
+
+    W, H = args.width, args.height
+    views = flythrough(args.views, W, H)
+    mode = pcr.MODE_LAS2 if args.mode == "las2" else pcr.MODE_LAS
+    prm = pcr.params(mode, time_stages=1)
+    fb = ctx.framebuffer(W, H)
+
+    # per-view algorithmic bytes of the raster kernel: headers of drawn batches,
+    # the planes their precision level needs, one level byte per batch
+    raster_bytes, frame_bytes, drawn_pts, hist = [], [], [], np.zeros(5, np.int64)
+    for v in views:
+        ctx.render(cloud, v, prm, fb)
+        ctx.resolve(cloud, prm, fb)
+        st = ctx.stats()
+        drawn = st.batches_total - st.batches_culled
+        plane = st.bytes_algorithmic - 64 * st.batches_total
+        raster_bytes.append(plane + 64 * drawn + st.batches_total)
+        # whole frame: select reads all headers; clear 8, resolve 8 + 4 gather + 4 write per pixel
+        frame_bytes.append(st.bytes_algorithmic + 24 * W * H)
+        drawn_pts.append(st.points_drawn)
+        hist += np.array(list(st.batches_per_level))
+    print(f"[bench] per-view stats done; level histogram {hist.tolist()}", file=sys.stderr, flush=True)
+
+    for k in range(args.warmup):
+        ctx.render(cloud, views[k % len(views)], prm, fb)
+        ctx.resolve(cloud, prm, fb)
+    ctx.synchronize()
+
+    K = args.steps
+    t0 = time.perf_counter()
+    for k in range(K):
+        ctx.render(cloud, views[k % len(views)], prm, fb)
+        ctx.resolve(cloud, prm, fb)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ft = ctx.frame_times(K)
+    print(f"[bench] timed {K} frames in {wall:.3f} s", file=sys.stderr, flush=True)
+
+    idx = [k % len(views) for k in range(K)]
+    raster_ms = ft[:, 2].astype(np.float64)
+    frame_ms = ft[:, 4].astype(np.float64)
+    achieved = sum(raster_bytes[i] for i in idx) / (raster_ms.sum() / 1e3) / 1e9
+    frame_gbs = sum(frame_bytes[i] for i in idx) / (frame_ms.sum() / 1e3) / 1e9
+    value = info.num_points * K / wall / 1e9
+
+    out = {
+        "metric": "Gpoints/s (1e9-pt adaptive-precision 1920x1080 frames)",
+        "value": round(value, 3),
+        "unit": "Gpoints/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "city-1e9-adaptive-1080p-flythrough", "points": info.num_points,
+                   "batches": info.num_batches, "batch_size": 10240, "width": W, "height": H,
+                   "views": len(views), "mode": args.mode},
+        "frame_ms_p50": round(float(np.percentile(frame_ms, 50)), 4),
+        "frame_ms_p99": round(float(np.percentile(frame_ms, 99)), 4),
+        "stage_ms_mean": {"clear": round(float(ft[:, 0].mean()), 4), "cull": round(float(ft[:, 1].mean()), 4),
+                          "raster": round(float(raster_ms.mean()), 4), "resolve": round(float(ft[:, 3].mean()), 4)},
+        "points_in_drawn_batches_mean": int(np.mean([drawn_pts[i] for i in idx])),
+        "level_histogram_per_frame": (hist / len(views)).round(1).tolist(),
+        "frame_algorithmic_GBs": round(frame_gbs, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_raster_batched<256,4>",
+                     "bytes_per_launch_mean": int(np.mean([raster_bytes[i] for i in idx]))},
+        "generate_s": round(gen_s, 2),
+    }
+    if args.cpu_frames > 0:
+        out["cpu_baseline"] = cpu_baseline(ctx, cloud, views, fb, prm, args.cpu_frames, info.num_points)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -186,6 +186,13 @@ class Context:
         check(capi.load().pcr_stats_get(self.h, C.byref(s)))
         return s
 
+    def frame_times(self, n: int) -> np.ndarray:
+        """[frames, 5] ms of clear, cull, raster, resolve, whole frame for the last n timed frames."""
+        out = np.zeros((n, 5), np.float32)
+        got = C.c_int()
+        check(capi.load().pcr_frame_times(self.h, n, ptr(out), C.byref(got)))
+        return out[:got.value]
+
     def lod_selection(self, cloud: Cloud) -> np.ndarray:
         n = cloud.info().num_batches
         out = np.zeros(n, np.uint8)

@@ -75,7 +75,9 @@ int pcr_create(int device, pcr_ctx** out) {
     PCR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     PCR_HIP(hipMalloc(&c->d_state, sizeof(FrameState)));
     PCR_HIP(hipMemset(c->d_state, 0, sizeof(FrameState)));
-    for (auto& e : c->ev) PCR_HIP(hipEventCreate(&e));
+    c->ring = new hipEvent_t[PCR_TIMING_RING][6];
+    for (int f = 0; f < PCR_TIMING_RING; f++)
+        for (int k = 0; k < 6; k++) PCR_HIP(hipEventCreate(&c->ring[f][k]));
     *out = c;
     return PCR_OK;
 }
@@ -84,7 +86,9 @@ int pcr_destroy(pcr_ctx* ctx) {
     if (!ctx) return PCR_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+    for (int f = 0; f < PCR_TIMING_RING; f++)
+        for (int k = 0; k < 6; k++) (void)hipEventDestroy(ctx->ring[f][k]);
+    delete[] ctx->ring;
     (void)hipFree(ctx->d_state);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -402,10 +406,11 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
               "render: block_size %d not in {64,128,256,512}", block);
     hipStream_t s = ctx->stream;
     ctx->timed = params->time_stages != 0;
-    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[0], s));
+    hipEvent_t* ev = ctx->ring[ctx->frames_timed % PCR_TIMING_RING];
+    if (ctx->timed) { ctx->frames_timed++; PCR_HIP(hipEventRecord(ev[0], s)); }
     int rc = PCR_OK;
     if (params->clear_before && (rc = clear_fb(s, fb))) return rc;
-    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[1], s));
+    if (ctx->timed) PCR_HIP(hipEventRecord(ev[1], s));
     if ((rc = launch_frame_reset(s, ctx->d_state))) return rc;
     if (!pts16) {
         rc = launch_select(s, cloud, *view, *params, ctx->d_state, mode == PCR_MODE_NODES ? 80.0f : 100.0f,
@@ -413,7 +418,7 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
         if (rc) return rc;
     }
     ctx->last_nb = pts16 ? 0 : cloud->num_batches;
-    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[2], s));
+    if (ctx->timed) PCR_HIP(hipEventRecord(ev[2], s));
     switch (mode) {
         case PCR_MODE_PLAIN16: rc = launch_raster_points16(s, cloud, *view, fb->d_keys); break;
         case PCR_MODE_LAS: rc = launch_raster_batched(s, cloud, *view, 1, block ? block : 128, ctx->d_state, fb->d_keys); break;
@@ -423,7 +428,7 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
         case PCR_MODE_HQS16: rc = launch_hqs_points16(s, cloud, *view, fb); break;
     }
     if (rc) return rc;
-    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[3], s));
+    if (ctx->timed) PCR_HIP(hipEventRecord(ev[3], s));
     return PCR_OK;
 }
 
@@ -433,10 +438,11 @@ int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params,
     const bool hqs = mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16;
     PCR_CHECK(hqs == (fb->kind == PCR_FB_HQS), "resolve: mode %d does not match the framebuffer kind", mode);
     PCR_CHECK(hqs || mode == PCR_MODE_PLAIN16 || cloud->d_rgba, "resolve: cloud has no colour plane");
-    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+    hipEvent_t* ev = ctx->ring[(ctx->frames_timed + PCR_TIMING_RING - 1) % PCR_TIMING_RING];
+    if (ctx->timed) PCR_HIP(hipEventRecord(ev[4], ctx->stream));
     int rc = launch_resolve(ctx->stream, cloud, mode, params->background, fb);
     if (rc) return rc;
-    if (ctx->timed) PCR_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
+    if (ctx->timed) PCR_HIP(hipEventRecord(ev[5], ctx->stream));
     return PCR_OK;
 }
 
@@ -451,15 +457,37 @@ int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out) {
     for (int i = 0; i < 5; i++) out->batches_per_level[i] = st.level_count[i];
     out->points_drawn = st.points_drawn;
     out->bytes_algorithmic = st.plane_bytes + 64ull * ctx->last_nb;
-    if (ctx->timed) {
+    if (ctx->timed && ctx->frames_timed) {
+        hipEvent_t* ev = ctx->ring[(ctx->frames_timed - 1) % PCR_TIMING_RING];
         float ms = 0;
-        if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) out->ms_clear = ms;
-        if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) out->ms_cull = ms;
-        if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]) == hipSuccess) out->ms_raster = ms;
-        if (hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]) == hipSuccess) out->ms_resolve = ms;
+        if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) out->ms_clear = ms;
+        if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) out->ms_cull = ms;
+        if (hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) out->ms_raster = ms;
+        if (hipEventElapsedTime(&ms, ev[4], ev[5]) == hipSuccess) out->ms_resolve = ms;
         (void)hipGetLastError();
     }
     ctx->last = *out;
+    return PCR_OK;
+}
+
+int pcr_frame_times(pcr_ctx* ctx, int n, float* out5, int* got) {
+    PCR_CHECK(ctx && out5 && got && n >= 0, "frame_times: bad argument");
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t avail = ctx->frames_timed < PCR_TIMING_RING ? ctx->frames_timed : PCR_TIMING_RING;
+    if ((uint64_t)n < avail) avail = (uint64_t)n;
+    for (uint64_t k = 0; k < avail; k++) {
+        hipEvent_t* ev = ctx->ring[(ctx->frames_timed - avail + k) % PCR_TIMING_RING];
+        const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {0, 5}};
+        for (int j = 0; j < 5; j++) {
+            float ms = -1.0f;
+            if (hipEventElapsedTime(&ms, ev[pairs[j][0]], ev[pairs[j][1]]) != hipSuccess) {
+                (void)hipGetLastError();
+                ms = -1.0f;
+            }
+            out5[5 * k + j] = ms;
+        }
+    }
+    *got = (int)avail;
     return PCR_OK;
 }
 

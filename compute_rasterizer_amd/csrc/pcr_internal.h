@@ -39,8 +39,9 @@ struct pcr_ctx {
     int cu_count = 0;
     hipStream_t stream = nullptr;
     pcr::FrameState* d_state = nullptr;   // one FrameState per context
-    hipEvent_t ev[8] = {};                // stage timers
-    bool timed = false;                   // last frame recorded stage events
+    hipEvent_t (*ring)[6] = nullptr;      // stage timers, PCR_TIMING_RING frames x 6 events
+    uint64_t frames_timed = 0;            // frames that recorded events (ring head)
+    bool timed = false;                   // the current frame records stage events
     pcr_stats last{};                     // host copy of the last frame's stats
     uint32_t last_nb = 0;
 };

@@ -263,6 +263,13 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view,
  * hqs/resolve.cs:62-129, compute_2021_earlyz/resolve.cs:41-78). */
 int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params, pcr_fb* fb);
 int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out);   /* synchronises */
+/* Stage times of the last n frames rendered with time_stages = 1, oldest
+ * first, five floats per frame: clear, cull, raster, resolve and the whole
+ * frame from clear to the end of resolve (ms). Up to
+ * PCR_TIMING_RING frames are kept (GLTimerQueries.cpp:55-153 kept 3 frames of
+ * GL timestamps for the same purpose). Synchronises; *got = frames written. */
+#define PCR_TIMING_RING 512
+int pcr_frame_times(pcr_ctx* ctx, int n, float* out5, int* got);
 
 /* ---- batched multi-view (persistent launch, hipGraph) ------------------- */
 typedef struct pcr_tune {
