@@ -78,7 +78,20 @@ def main():
     ap.add_argument("--views", type=int, default=64)
     ap.add_argument("--mode", default="las2", choices=["las", "las2"])
     ap.add_argument("--cpu-frames", type=int, default=2, help="0 skips the CPU baseline")
+    ap.add_argument("--tile-px", type=int, default=0, help="LDS tile size of the tiled raster (0 = direct)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 4 device copies of 1 GiB (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 << 28, "copies": 4}))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -98,7 +111,7 @@ def main():
     W, H = args.width, args.height
     views = flythrough(args.views, W, H)
     mode = pcr.MODE_LAS2 if args.mode == "las2" else pcr.MODE_LAS
-    prm = pcr.params(mode, time_stages=1)
+    prm = pcr.params(mode, time_stages=1, tile_px=args.tile_px)
     fb = ctx.framebuffer(W, H)
 
     # per-view algorithmic bytes of the raster kernel: headers of drawn batches,
@@ -154,7 +167,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "city-1e9-adaptive-1080p-flythrough", "points": info.num_points,
                    "batches": info.num_batches, "batch_size": 10240, "width": W, "height": H,
-                   "views": len(views), "mode": args.mode},
+                   "views": len(views), "mode": args.mode, "tile_px": args.tile_px},
         "frame_ms_p50": round(float(np.percentile(frame_ms, 50)), 4),
         "frame_ms_p99": round(float(np.percentile(frame_ms, 99)), 4),
         "stage_ms_mean": {"clear": round(float(ft[:, 0].mean()), 4), "cull": round(float(ft[:, 1].mean()), 4),
@@ -164,7 +177,7 @@ def main():
         "frame_algorithmic_GBs": round(frame_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_raster_batched<256,4>",
+                     "kernel": "k_raster_tiled<256,%d>" % args.tile_px if args.tile_px else "k_raster_batched<256,4>",
                      "bytes_per_launch_mean": int(np.mean([raster_bytes[i] for i in idx]))},
         "generate_s": round(gen_s, 2),
     }

@@ -32,10 +32,11 @@ def look_at(eye, target, up=(0.0, 1.0, 0.0), fovy=60.0, near=0.1, far=1000.0, or
 
 
 def params(mode=MODE_LAS2, culling=1, background=None, lod_rule=LOD_SPACING, lod_px=1.0, block_size=0,
-           clear_before=1, time_stages=0) -> RenderParams:
+           clear_before=1, time_stages=0, tile_px=0) -> RenderParams:
     if background is None:
         background = BG_BLACK if mode in (MODE_PLAIN16,) else BG_LAS
-    return RenderParams(mode, culling, lod_rule, lod_px, background, block_size, clear_before, time_stages)
+    return RenderParams(mode, culling, lod_rule, lod_px, background, block_size, clear_before, time_stages,
+                        tile_px)
 
 
 def gen_cfg(scene, n, seed=1, extent=2000.0, amplitude=150.0, jitter=0.5, num_buildings=256,
@@ -52,6 +53,22 @@ def encode_batches_host(xyz: np.ndarray, batch_size: int = POINTS_PER_BATCH):
     lo, med, hi = (np.zeros(n, np.uint32) for _ in range(3))
     check(capi.load().pcr_encode_batches_host(ptr(xyz), n, batch_size, ptr(b), ptr(lo), ptr(med), ptr(hi)))
     return b, lo, med, hi
+
+
+def octree_layout_host(xyz, rgba, seed=1, root_spacing_div=128.0, max_node_points=20000, max_depth=12):
+    """Potree-style node layout on the host: returns (nodes, lo, med, hi, rgba)."""
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    rgba = np.ascontiguousarray(rgba, np.uint32)
+    o = OctreeOpts(seed, root_spacing_div, max_node_points, max_depth)
+    nn, ns = C.c_uint32(0), C.c_uint64(0)
+    lib = capi.load()
+    check(lib.pcr_octree_layout_host(ptr(xyz), ptr(rgba), xyz.shape[0], C.byref(o), None, C.byref(nn), None, None,
+                                     None, None, C.byref(ns)))
+    nodes = np.zeros(nn.value, BATCH_DTYPE)
+    lo, med, hi, col = (np.zeros(ns.value, np.uint32) for _ in range(4))
+    check(lib.pcr_octree_layout_host(ptr(xyz), ptr(rgba), xyz.shape[0], C.byref(o), ptr(nodes), C.byref(nn),
+                                     ptr(lo), ptr(med), ptr(hi), ptr(col), C.byref(ns)))
+    return nodes, lo, med, hi, col
 
 
 class Framebuffer:
@@ -130,6 +147,101 @@ class Cloud:
             self.close()
         except Exception:
             pass
+
+
+class MultiView:
+    """One persistent launch rendering many views of a cloud (pcr_multiview)."""
+
+    def __init__(self, ctx: "Context", cloud: Cloud, views, fbs, prm: RenderParams, tune: Tune | None = None,
+                 use_graph=True):
+        self.ctx, self.cloud, self.fbs = ctx, cloud, list(fbs)
+        n = len(views)
+        self._views = (View * n)(*views)
+        self._fbs = (C.c_void_p * n)(*[f.h.value for f in self.fbs])
+        h = C.c_void_p()
+        check(capi.load().pcr_multiview_create(ctx.h, cloud.h, self._views, n, self._fbs, C.byref(prm),
+                                               C.byref(tune) if tune is not None else None, int(use_graph),
+                                               C.byref(h)))
+        self.h = h
+
+    def set_views(self, views):
+        n = len(views)
+        self._views = (View * n)(*views)
+        check(capi.load().pcr_multiview_set_views(self.h, self._views, n))
+
+    def launch(self):
+        check(capi.load().pcr_multiview_launch(self.h))
+
+    def close(self):
+        if self.h:
+            capi.load().pcr_multiview_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching render server (pcr_server)."""
+
+    def __init__(self, ctx: "Context", budget_bytes: int, max_batch=16, num_streams=4):
+        h = C.c_void_p()
+        check(capi.load().pcr_server_create(ctx.h, budget_bytes, max_batch, num_streams, C.byref(h)))
+        self.ctx, self.h, self.clouds = ctx, h, []
+
+    def add_cloud(self, cloud: Cloud) -> int:
+        cid = C.c_int()
+        check(capi.load().pcr_server_add_cloud(self.h, cloud.h, C.byref(cid)))
+        self.clouds.append(cloud)
+        return cid.value
+
+    def submit(self, cloud_id: int, view: View, mode=MODE_LAS2, lod_px=1.0) -> int:
+        req = capi.Request(cloud_id, mode, lod_px, 0, view)
+        t = C.c_uint64()
+        check(capi.load().pcr_server_submit(self.h, C.byref(req), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int, npix: int):
+        """(image, latency_ms) or None while the request is in flight."""
+        out = np.empty(npix, np.uint32)
+        lat = C.c_double()
+        rc = capi.load().pcr_server_poll(self.h, ticket, ptr(out), C.byref(lat))
+        if rc == capi.PCR_E_NOT_READY:
+            return None
+        check(rc)
+        return out, lat.value
+
+    def wait(self, ticket: int, npix: int):
+        out = np.empty(npix, np.uint32)
+        lat = C.c_double()
+        check(capi.load().pcr_server_wait(self.h, ticket, ptr(out), C.byref(lat)))
+        return out, lat.value
+
+    def cancel(self, ticket: int):
+        check(capi.load().pcr_server_cancel(self.h, ticket))
+
+    def stats(self) -> capi.ServerStats:
+        s = capi.ServerStats()
+        check(capi.load().pcr_server_stats_get(self.h, C.byref(s)))
+        return s
+
+    def close(self):
+        if self.h:
+            capi.load().pcr_server_destroy(self.h)
+            self.h = None
+
+
+def autotune(ctx: "Context", cloud: Cloud, views, fbs, prm: RenderParams, iters=5):
+    """Sweep block size x points per thread for a multi-view frame: (Tune, ms)."""
+    n = len(views)
+    vs = (View * n)(*views)
+    fs = (C.c_void_p * n)(*[f.h.value for f in fbs])
+    best, ms = Tune(), C.c_float()
+    check(capi.load().pcr_autotune(ctx.h, cloud.h, vs, n, fs, C.byref(prm), iters, C.byref(best), C.byref(ms)))
+    return best, ms.value
 
 
 class Context:

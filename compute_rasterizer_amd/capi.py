@@ -59,7 +59,8 @@ class View(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("mode", C.c_int32), ("frustum_culling", C.c_int32), ("lod_rule", C.c_int32),
                 ("lod_px", C.c_float), ("background", C.c_uint32), ("block_size", C.c_int32),
-                ("clear_before", C.c_int32), ("time_stages", C.c_int32)]
+                ("clear_before", C.c_int32), ("time_stages", C.c_int32), ("tile_px", C.c_int32),
+                ("pad", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):
@@ -131,6 +132,8 @@ SIGNATURES = {
     "pcr_cloud_build_octree_host": (C.c_int, [P, P, P, u64, C.POINTER(OctreeOpts), PP]),
     "pcr_cloud_build_octree_device": (C.c_int, [P, P, P, u64, C.POINTER(OctreeOpts), PP]),
     "pcr_lod_selection": (C.c_int, [P, P, P, u32]),
+    "pcr_octree_layout_host": (C.c_int, [P, P, u64, C.POINTER(OctreeOpts), P, C.POINTER(u32), P, P, P, P,
+                                         C.POINTER(u64)]),
     "pcr_fb_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, PP]),
     "pcr_fb_destroy": (C.c_int, [P]),
     "pcr_fb_clear": (C.c_int, [P, P]),

@@ -401,6 +401,8 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
               "render: cloud has no %s layout for mode %d", pts16 ? "16 B/pt" : "batched", mode);
     PCR_CHECK(mode != PCR_MODE_NODES || params->lod_rule == PCR_LOD_SPACING || params->lod_rule == PCR_LOD_SIZE80,
               "render: bad lod_rule %d", params->lod_rule);
+    PCR_CHECK(params->tile_px == 0 || params->tile_px == 1024 || params->tile_px == 2048 || params->tile_px == 4096,
+              "render: tile_px %d not in {0,1024,2048,4096}", params->tile_px);
     const int block = params->block_size;
     PCR_CHECK(block == 0 || block == 64 || block == 128 || block == 256 || block == 512,
               "render: block_size %d not in {64,128,256,512}", block);
@@ -423,7 +425,10 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
         case PCR_MODE_PLAIN16: rc = launch_raster_points16(s, cloud, *view, fb->d_keys); break;
         case PCR_MODE_LAS: rc = launch_raster_batched(s, cloud, *view, 1, block ? block : 128, ctx->d_state, fb->d_keys); break;
         case PCR_MODE_LAS2:
-        case PCR_MODE_NODES: rc = launch_raster_batched(s, cloud, *view, 4, block ? block : 256, ctx->d_state, fb->d_keys); break;
+        case PCR_MODE_NODES:
+            rc = params->tile_px ? launch_raster_tiled(s, cloud, *view, params->tile_px, fb->d_keys)
+                                 : launch_raster_batched(s, cloud, *view, 4, block ? block : 256, ctx->d_state, fb->d_keys);
+            break;
         case PCR_MODE_HQS: rc = launch_hqs_batched(s, cloud, *view, ctx->d_state, fb); break;
         case PCR_MODE_HQS16: rc = launch_hqs_points16(s, cloud, *view, fb); break;
     }

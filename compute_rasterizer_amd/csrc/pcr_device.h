@@ -130,6 +130,46 @@ __device__ __forceinline__ bool project(const float* __restrict__ t, float x, fl
     return true;
 }
 
+// Same as project() but returning the pixel column and row.
+__device__ __forceinline__ bool project_xy(const float* __restrict__ t, float x, float y, float z, int w, int h,
+                                           int& ix, int& iy, uint32_t& depth_bits) {
+    const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
+    const float py = fma_(t[9], z, fma_(t[5], y, fma_(t[1], x, t[13])));
+    const float pw = fma_(t[11], z, fma_(t[7], y, fma_(t[3], x, t[15])));
+    const float nx = px / pw, ny = py / pw;
+    if (!(pw > 0.0f && nx >= -1.0f && nx <= 1.0f && ny >= -1.0f && ny <= 1.0f)) return false;
+    ix = (int)(fma_(nx, 0.5f, 0.5f) * (float)w);
+    iy = (int)(fma_(ny, 0.5f, 0.5f) * (float)h);
+    if (ix >= w || iy >= h) return false;
+    depth_bits = __float_as_uint(pw);
+    return true;
+}
+
+// Screen rectangle (inclusive, clamped to the image, 1-pixel margin) covered
+// by the projection of an AABB; false when a corner is not in front of the
+// eye or the rectangle is empty. It only routes points to a tile: a point
+// whose pixel falls outside it takes the direct path, so the result never
+// depends on this bound being conservative.
+__device__ inline bool screen_rect(const float* __restrict__ t, const float mn[3], const float mx[3], int w, int h,
+                                   int& x0, int& y0, int& x1, int& y1) {
+    float sx0 = 3.0e38f, sx1 = -3.0e38f, sy0 = 3.0e38f, sy1 = -3.0e38f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const float x = (c & 1) ? mx[0] : mn[0], y = (c & 2) ? mx[1] : mn[1], z = (c & 4) ? mx[2] : mn[2];
+        const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
+        const float py = fma_(t[9], z, fma_(t[5], y, fma_(t[1], x, t[13])));
+        const float pw = fma_(t[11], z, fma_(t[7], y, fma_(t[3], x, t[15])));
+        if (!(pw > 0.0f)) return false;
+        const float sx = fma_(px / pw, 0.5f, 0.5f) * (float)w, sy = fma_(py / pw, 0.5f, 0.5f) * (float)h;
+        sx0 = fminf(sx0, sx); sx1 = fmaxf(sx1, sx); sy0 = fminf(sy0, sy); sy1 = fmaxf(sy1, sy);
+    }
+    sx0 = fmaxf(sx0, -2.0f); sy0 = fmaxf(sy0, -2.0f);
+    sx1 = fminf(sx1, (float)w + 2.0f); sy1 = fminf(sy1, (float)h + 2.0f);
+    x0 = max((int)floorf(sx0) - 1, 0); y0 = max((int)floorf(sy0) - 1, 0);
+    x1 = min((int)floorf(sx1) + 1, w - 1); y1 = min((int)floorf(sy1) + 1, h - 1);
+    return x1 >= x0 && y1 >= y0;
+}
+
 // 64-bit fetch-min with the benign early-out compare of render.cs:423-426.
 // The plain load can only be stale-high within a frame (values only
 // decrease), so skipping when key >= old never loses a minimum.

@@ -18,6 +18,13 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ u32x4 ld_stream(const u32x4* p) { return __builtin_nontemporal_load(p); }
 
+// 64-bit atomicMin on LDS (ds_min_u64): the pointer is cast to the LDS
+// address space so that the compiler does not emit a flat atomic.
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+__device__ __forceinline__ void lds_min(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_min((lds_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -214,6 +221,119 @@ template <int BLOCK, int VEC>
 static void raster_batched_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb) {
     k_raster_batched<BLOCK, VEC><<<c->num_batches, BLOCK, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
                                                                   c->d_hi, v, fb);
+}
+
+// ------------------------------------------------------------ tiled raster
+// A batch whose projected AABB covers at most TILE_PIX pixels first merges
+// its points into an LDS copy of that screen rectangle with 64-bit LDS
+// atomicMin, then flushes the touched pixels to the framebuffer with the
+// usual compare + global atomicMin. min is associative and commutative, so
+// the framebuffer is bit-identical to the direct path; the per-point global
+// traffic drops to one access per touched pixel per batch.
+template <int LC, bool TILE>
+__device__ __forceinline__ void tile_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi, uint32_t local,
+                                           uint32_t np, uint32_t first, const pcr_view& v,
+                                           unsigned long long* __restrict__ fb, unsigned long long* tile, int x0,
+                                           int y0, int tw, int th) {
+    if (local >= np) return;
+    float p[3];
+    decode<LC>(d, lo, med, hi, p);
+    int ix, iy;
+    uint32_t depth;
+    if (!project_xy(v.transform, p[0], p[1], p[2], v.width, v.height, ix, iy, depth)) return;
+    const unsigned long long key = ((unsigned long long)depth << 32) | (first + local);
+    const uint32_t ux = (uint32_t)(ix - x0), uy = (uint32_t)(iy - y0);
+    if (TILE && ux < (uint32_t)tw && uy < (uint32_t)th) lds_min(tile + uy * (uint32_t)tw + ux, key);
+    else fb_min(fb, (uint32_t)ix + (uint32_t)iy * (uint32_t)v.width, key);
+}
+
+template <int BLOCK, int LC, bool TILE>
+__device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __restrict__ lo,
+                                          const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                          uint32_t first, uint32_t np, const pcr_view& v,
+                                          unsigned long long* __restrict__ fb, unsigned long long* tile, int x0,
+                                          int y0, int tw, int th) {
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + first);
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + first);
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + first);
+    const uint32_t nq = (np + 3u) >> 2;
+    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
+        const u32x4 a = ld_stream(lo4 + q);
+        const u32x4 m = LC <= 1 ? ld_stream(med4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 c = LC == 0 ? ld_stream(hi4 + q) : u32x4{0, 0, 0, 0};
+        const uint32_t base = q * 4u;
+        tile_point<LC, TILE>(d, a.x, m.x, c.x, base + 0, np, first, v, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE>(d, a.y, m.y, c.y, base + 1, np, first, v, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE>(d, a.z, m.z, c.z, base + 2, np, first, v, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE>(d, a.w, m.w, c.w, base + 3, np, first, v, fb, tile, x0, y0, tw, th);
+    }
+}
+
+template <int BLOCK, bool TILE>
+__device__ __forceinline__ void tile_dispatch(int level, const Decode& d, const uint32_t* __restrict__ lo,
+                                              const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                              const pcr_batch& h, const pcr_view& v, unsigned long long* __restrict__ fb,
+                                              unsigned long long* tile, int x0, int y0, int tw, int th) {
+    if (level == 0) tile_loop<BLOCK, 0, TILE>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
+    else if (level == 1) tile_loop<BLOCK, 1, TILE>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
+    else tile_loop<BLOCK, 2, TILE>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
+}
+
+template <int BLOCK, int TILE_PIX>
+__global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restrict__ b,
+                                                        const uint8_t* __restrict__ levels,
+                                                        const uint32_t* __restrict__ lo,
+                                                        const uint32_t* __restrict__ med,
+                                                        const uint32_t* __restrict__ hi, pcr_view v,
+                                                        unsigned long long* __restrict__ fb) {
+    __shared__ unsigned long long tile[TILE_PIX];
+    const uint32_t bi = blockIdx.x;
+    const int level = levels[bi];
+    if (level == kCulled) return;
+    const pcr_batch h = b[bi];
+    const Decode d = make_decode(level, h.min, h.max);
+    int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
+    bool use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1);
+    const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
+    use = use && tw <= TILE_PIX && th <= TILE_PIX && tw * th <= TILE_PIX;
+    // every thread evaluates `use` from the same header and view, so the
+    // branch (and the barriers inside it) is uniform across the workgroup
+    if (use) {
+        const int n = tw * th;
+        for (int i = threadIdx.x; i < n; i += BLOCK) tile[i] = PCR_FB_CLEAR;
+        __syncthreads();
+        tile_dispatch<BLOCK, true>(level, d, lo, med, hi, h, v, fb, tile, x0, y0, tw, th);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += BLOCK) {
+            const unsigned long long k = tile[i];
+            if (k != PCR_FB_CLEAR) {
+                const int uy = i / tw, ux = i - uy * tw;
+                fb_min(fb, (uint32_t)(x0 + ux) + (uint32_t)(y0 + uy) * (uint32_t)v.width, k);
+            }
+        }
+    } else {
+        tile_dispatch<BLOCK, false>(level, d, lo, med, hi, h, v, fb, nullptr, 0, 0, 0, 0);
+    }
+}
+
+int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, unsigned long long* fb) {
+    if (c->num_batches == 0) return PCR_OK;
+    switch (tile_px) {
+        case 1024:
+            k_raster_tiled<256, 1024><<<c->num_batches, 256, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
+                                                                     c->d_hi, v, fb);
+            break;
+        case 2048:
+            k_raster_tiled<256, 2048><<<c->num_batches, 256, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
+                                                                     c->d_hi, v, fb);
+            break;
+        default:
+            k_raster_tiled<256, 4096><<<c->num_batches, 256, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
+                                                                     c->d_hi, v, fb);
+            break;
+    }
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
 }
 
 int launch_raster_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int vec, int block,

@@ -1,0 +1,415 @@
+// pcr_multiview.hip — many views of one cloud in one persistent launch.
+//
+// The reference renders one view per dispatch; its VR methods decode a batch
+// once and project it into two views in the same dispatch
+// (modules/compute_loop_las_hqs_vr/color.cs:403-510). This generalises that
+// to up to 64 views of uneven resolution:
+//   k_mv_select  one thread per batch: per-view cull + precision level
+//                (levels[v][b], 255 = not drawn); batches kept by any view
+//                are appended to a work list with their most precise level.
+//   k_mv_raster  persistent workgroups take (batch, chunk) items from an
+//                atomic queue; each item's planes are read once, at the most
+//                precise level any view needs, and every view decodes the
+//                words at its own level, so each framebuffer is bit-identical
+//                to a single-view render of that view.
+//   k_mv_clear / k_mv_resolve cover all views in one launch each.
+// The frame (clear, reset, select, raster, resolve) is captured once in a
+// hipGraph; views change through a device buffer copied before each launch.
+#include <algorithm>
+#include <vector>
+#include "pcr_device.h"
+#include "pcr_internal.h"
+
+namespace pcr {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kMaxViews = 64;
+
+struct MVState {
+    uint32_t list_count;
+    uint32_t queue_head;
+};
+
+__global__ void k_mv_reset(MVState* st) {
+    if (threadIdx.x == 0) { st->list_count = 0; st->queue_head = 0; }
+}
+
+__global__ void k_mv_clear(unsigned long long* const* __restrict__ fbs, const pcr_view* __restrict__ views) {
+    const uint32_t v = blockIdx.y;
+    const uint64_t npix = (uint64_t)views[v].width * views[v].height;
+    unsigned long long* fb = fbs[v];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (uint64_t)gridDim.x * blockDim.x)
+        fb[i] = PCR_FB_CLEAR;
+}
+
+// lod != 0: LOD node clouds; a node is drawn for view v when it and its
+// ancestors pass the view's frustum and every ancestor's projected spacing
+// exceeds lod_px[v] (or, rule SIZE80, its projected size is >= 80 px).
+__global__ __launch_bounds__(256) void k_mv_select(const pcr_batch* __restrict__ b, uint32_t nb,
+                                                   const pcr_view* __restrict__ views, int nviews, int culling,
+                                                   int lod, int rule, const float* __restrict__ lod_px,
+                                                   uint8_t* __restrict__ levels, uint8_t* __restrict__ best,
+                                                   uint32_t* __restrict__ list, MVState* __restrict__ st) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int top = kCulled;
+    if (i < nb) {
+        const pcr_batch h = b[i];
+        const float first = lod ? 80.0f : 100.0f;
+        for (int v = 0; v < nviews; v++) {
+            const pcr_view& vw = views[v];
+            bool vis = !culling || frustum_visible(vw.transform_frustum, h.min, h.max);
+            if (lod && vis) {
+                if (rule == PCR_LOD_SIZE80) {
+                    vis = precision_level(h.min, h.max, vw, 80.0f) < 4;
+                } else {
+                    int32_t p = h.parent;
+                    while (vis && p >= 0) {
+                        const pcr_batch a = b[p];
+                        vis = (!culling || frustum_visible(vw.transform_frustum, a.min, a.max)) &&
+                              projected_px(a.min, a.max, a.spacing, vw.view_world, vw.proj, vw.width, vw.height) >
+                                  lod_px[v];
+                        p = a.parent;
+                    }
+                }
+            }
+            const int level = vis ? precision_level(h.min, h.max, vw, first) : kCulled;
+            levels[(uint64_t)v * nb + i] = (uint8_t)level;
+            top = level < top ? level : top;
+        }
+        best[i] = (uint8_t)top;
+    }
+    const bool keep = top != kCulled;
+    const unsigned long long m = __ballot(keep);
+    if (m) {
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&st->list_count, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader);
+        if (keep) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    }
+}
+
+template <int LC>
+__device__ __forceinline__ void mv_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi, uint32_t local,
+                                         uint32_t cnt, uint32_t index, const pcr_view& v,
+                                         unsigned long long* __restrict__ fb) {
+    if (local >= cnt) return;
+    float p[3];
+    decode<LC>(d, lo, med, hi, p);
+    uint32_t pix, depth;
+    if (!project(v.transform, p[0], p[1], p[2], v.width, v.height, pix, depth)) return;
+    fb_min(fb, pix, ((unsigned long long)depth << 32) | index);
+}
+
+template <int LC>
+__device__ __forceinline__ void mv_points4(const Decode& d, const u32x4& a, const u32x4& m, const u32x4& c,
+                                           uint32_t base, uint32_t cnt, uint32_t first, const pcr_view& v,
+                                           unsigned long long* __restrict__ fb) {
+    mv_point<LC>(d, a.x, m.x, c.x, base + 0, cnt, first + base + 0, v, fb);
+    mv_point<LC>(d, a.y, m.y, c.y, base + 1, cnt, first + base + 1, v, fb);
+    mv_point<LC>(d, a.z, m.z, c.z, base + 2, cnt, first + base + 2, v, fb);
+    mv_point<LC>(d, a.w, m.w, c.w, base + 3, cnt, first + base + 3, v, fb);
+}
+
+// Persistent raster over (batch, chunk) items; chunk_pts is a multiple of 4.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_mv_raster(const pcr_batch* __restrict__ b, uint32_t nb,
+                                                     const uint8_t* __restrict__ levels,
+                                                     const uint8_t* __restrict__ best,
+                                                     const uint32_t* __restrict__ list, MVState* __restrict__ st,
+                                                     const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
+                                                     const uint32_t* __restrict__ hi,
+                                                     const pcr_view* __restrict__ views, int nviews,
+                                                     unsigned long long* const* __restrict__ fbs, uint32_t chunk_pts,
+                                                     uint32_t chunks_per_batch) {
+    __shared__ uint32_t s_item;
+    __shared__ uint8_t s_vid[kMaxViews];
+    __shared__ uint8_t s_lv[kMaxViews];
+    __shared__ uint32_t s_nv;
+    const uint32_t total = st->list_count * chunks_per_batch;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&st->queue_head, 1u);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= total) break;
+        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[item / chunks_per_batch]);
+        const uint32_t chunk = item % chunks_per_batch;
+        // compact the views that draw this batch (wave 0, one lane per view)
+        if (threadIdx.x < 64) {
+            const int v = (int)threadIdx.x;
+            const int lv = v < nviews ? levels[(uint64_t)v * nb + bi] : kCulled;
+            const unsigned long long m = __ballot(lv != kCulled);
+            if (lv != kCulled) {
+                const uint32_t pos = (uint32_t)__popcll(m & ((1ull << v) - 1ull));
+                s_vid[pos] = (uint8_t)v;
+                s_lv[pos] = (uint8_t)lv;
+            }
+            if (v == 0) s_nv = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        const pcr_batch h = b[bi];
+        const uint32_t start = chunk * chunk_pts;
+        const uint32_t nv = __builtin_amdgcn_readfirstlane(s_nv);
+        if (start < h.num_points) {
+            const uint32_t cnt = min(chunk_pts, h.num_points - start);
+            const uint32_t first = h.first_point + start;
+            const int top = best[bi];
+            const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + first);
+            const u32x4* med4 = reinterpret_cast<const u32x4*>(med + first);
+            const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + first);
+            const uint32_t nq = (cnt + 3u) >> 2;
+            for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
+                const u32x4 a = __builtin_nontemporal_load(lo4 + q);
+                const u32x4 m = top <= 1 ? __builtin_nontemporal_load(med4 + q) : u32x4{0, 0, 0, 0};
+                const u32x4 c = top == 0 ? __builtin_nontemporal_load(hi4 + q) : u32x4{0, 0, 0, 0};
+                for (uint32_t k = 0; k < nv; k++) {
+                    const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
+                    const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
+                    const pcr_view& v = views[vi];
+                    const Decode d = make_decode(lv, h.min, h.max);
+                    unsigned long long* fb = fbs[vi];
+                    if (lv == 0) mv_points4<0>(d, a, m, c, q * 4u, cnt, first, v, fb);
+                    else if (lv == 1) mv_points4<1>(d, a, m, c, q * 4u, cnt, first, v, fb);
+                    else mv_points4<2>(d, a, m, c, q * 4u, cnt, first, v, fb);
+                }
+            }
+        }
+        __syncthreads();   // s_item / s_vid are rewritten by the next item
+    }
+}
+
+__global__ void k_mv_resolve(const unsigned long long* const* __restrict__ fbs, uint32_t* const* __restrict__ out,
+                             const pcr_view* __restrict__ views, const uint32_t* __restrict__ rgba, uint64_t npoints,
+                             uint32_t bg) {
+    const uint32_t v = blockIdx.y;
+    const uint64_t npix = (uint64_t)views[v].width * views[v].height;
+    const unsigned long long* fb = fbs[v];
+    uint32_t* o = out[v];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long k = fb[i];
+        const uint32_t id = (uint32_t)k;
+        o[i] = (k == PCR_FB_CLEAR || id >= npoints) ? bg : rgba[id];
+    }
+}
+
+int mv_alloc(MVBuffers& B, int nviews, uint64_t nb) {
+    hipError_t e = hipMalloc(&B.d_views, sizeof(pcr_view) * nviews);
+    if (e == hipSuccess) e = hipMalloc(&B.d_fbs, sizeof(void*) * nviews);
+    if (e == hipSuccess) e = hipMalloc(&B.d_out, sizeof(void*) * nviews);
+    if (e == hipSuccess) e = hipMalloc(&B.d_lod, sizeof(float) * nviews);
+    if (e == hipSuccess) e = hipMalloc(&B.d_levels, (nb ? nb : 1) * nviews);
+    if (e == hipSuccess) e = hipMalloc(&B.d_best, nb ? nb : 1);
+    if (e == hipSuccess) e = hipMalloc(&B.d_list, (nb ? nb : 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&B.d_state, sizeof(MVState));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        mv_release(B);
+        set_error("multiview: device allocation for %d views x %llu batches failed", nviews, (unsigned long long)nb);
+        return PCR_E_OOM;
+    }
+    B.cap_views = nviews;
+    B.cap_batches = nb;
+    return PCR_OK;
+}
+
+void mv_release(MVBuffers& B) {
+    (void)hipFree(B.d_views); (void)hipFree(B.d_fbs); (void)hipFree(B.d_out); (void)hipFree(B.d_lod);
+    (void)hipFree(B.d_levels); (void)hipFree(B.d_best); (void)hipFree(B.d_list); (void)hipFree(B.d_state);
+    B = MVBuffers{};
+}
+
+// Queues one multi-view frame: clear, reset, select, raster, resolve.
+int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, uint64_t max_pix,
+                   const pcr_render_params& prm, bool lod, const pcr_tune& t, int groups) {
+    uint32_t maxnp = 4;
+    for (const pcr_batch& h : c->h_batches) maxnp = maxnp > h.num_points ? maxnp : h.num_points;
+    const uint32_t chunk_pts = (uint32_t)(t.block_size * t.points_per_thread);
+    const uint32_t cpb = (maxnp + chunk_pts - 1) / chunk_pts;
+    const uint32_t pix_blocks = (uint32_t)std::min<uint64_t>((max_pix + 255) / 256, 4096);
+    dim3 g2(pix_blocks, (uint32_t)nviews);
+    k_mv_clear<<<g2, 256, 0, s>>>(B.d_fbs, B.d_views);
+    k_mv_reset<<<1, 64, 0, s>>>(B.d_state);
+    k_mv_select<<<(c->num_batches + 255) / 256, 256, 0, s>>>(c->d_batches, c->num_batches, B.d_views, nviews,
+                                                             prm.frustum_culling, lod ? 1 : 0, prm.lod_rule, B.d_lod,
+                                                             B.d_levels, B.d_best, B.d_list, B.d_state);
+#define MV_RASTER(BS)                                                                                           \
+    k_mv_raster<BS><<<groups, BS, 0, s>>>(c->d_batches, c->num_batches, B.d_levels, B.d_best, B.d_list,          \
+                                          B.d_state, c->d_lo, c->d_med, c->d_hi, B.d_views, nviews, B.d_fbs,    \
+                                          chunk_pts, cpb)
+    if (t.block_size == 64) MV_RASTER(64);
+    else if (t.block_size == 128) MV_RASTER(128);
+    else if (t.block_size == 512) MV_RASTER(512);
+    else MV_RASTER(256);
+#undef MV_RASTER
+    k_mv_resolve<<<g2, 256, 0, s>>>(B.d_fbs, B.d_out, B.d_views, c->d_rgba, c->num_points, prm.background);
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+}  // namespace pcr
+
+using namespace pcr;
+
+struct pcr_multiview {
+    pcr_ctx* ctx = nullptr;
+    pcr_cloud* cloud = nullptr;
+    int nviews = 0;
+    pcr_render_params prm{};
+    pcr_tune tune{};
+    bool lod = false;
+    std::vector<pcr_view> h_views;
+    MVBuffers B;
+    uint64_t max_pix = 0;
+    int groups = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace pcr {
+
+static int mv_enqueue(pcr_multiview* mv, hipStream_t s) {
+    return mv_enqueue_raw(s, mv->cloud, mv->B, mv->nviews, mv->max_pix, mv->prm, mv->lod, mv->tune, mv->groups);
+}
+
+static void mv_free(pcr_multiview* mv) {
+    if (mv->exec) (void)hipGraphExecDestroy(mv->exec);
+    if (mv->graph) (void)hipGraphDestroy(mv->graph);
+    mv_release(mv->B);
+}
+
+}  // namespace pcr
+
+extern "C" {
+
+int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nviews, pcr_fb* const* fbs,
+                         const pcr_render_params* params, const pcr_tune* tune, int use_graph, pcr_multiview** out) {
+    PCR_CHECK(ctx && cloud && views && fbs && params && out, "multiview_create: null argument");
+    PCR_CHECK(nviews >= 1 && nviews <= kMaxViews, "multiview_create: %d views (1..%d)", nviews, kMaxViews);
+    PCR_CHECK(cloud->d_batches, "multiview_create: cloud has no batched layout");
+    PCR_CHECK(params->mode == PCR_MODE_LAS || params->mode == PCR_MODE_LAS2 || params->mode == PCR_MODE_NODES,
+              "multiview_create: mode %d is not a batched key mode (LAS, LAS2, NODES)", params->mode);
+    for (int v = 0; v < nviews; v++) {
+        PCR_CHECK(fbs[v] && fbs[v]->kind == PCR_FB_KEY64, "multiview_create: view %d needs a KEY64 framebuffer", v);
+        PCR_CHECK(views[v].width == fbs[v]->width && views[v].height == fbs[v]->height,
+                  "multiview_create: view %d is %dx%d but its framebuffer is %dx%d", v, views[v].width,
+                  views[v].height, fbs[v]->width, fbs[v]->height);
+    }
+    pcr_tune t = tune ? *tune : pcr_tune{256, 40, 0};
+    if (t.block_size == 0) t.block_size = 256;
+    if (t.points_per_thread == 0) t.points_per_thread = 40;
+    PCR_CHECK(t.block_size == 64 || t.block_size == 128 || t.block_size == 256 || t.block_size == 512,
+              "multiview_create: block_size %d not in {64,128,256,512}", t.block_size);
+    PCR_CHECK(t.points_per_thread > 0 && t.points_per_thread % 4 == 0 && t.points_per_thread <= 1024,
+              "multiview_create: points_per_thread %d must be a positive multiple of 4", t.points_per_thread);
+    pcr_multiview* mv = new pcr_multiview();
+    mv->ctx = ctx; mv->cloud = cloud; mv->nviews = nviews; mv->prm = *params; mv->tune = t;
+    mv->lod = params->mode == PCR_MODE_NODES;
+    mv->h_views.assign(views, views + nviews);
+    mv->groups = t.persistent_groups > 0 ? t.persistent_groups : ctx->cu_count * (2048 / t.block_size);
+    std::vector<unsigned long long*> hf(nviews);
+    std::vector<uint32_t*> ho(nviews);
+    std::vector<float> hl(nviews, params->lod_px);
+    for (int v = 0; v < nviews; v++) {
+        hf[v] = fbs[v]->d_keys;
+        ho[v] = fbs[v]->d_rgba;
+        mv->max_pix = std::max<uint64_t>(mv->max_pix, (uint64_t)views[v].width * views[v].height);
+    }
+    int rc0 = mv_alloc(mv->B, nviews, cloud->num_batches);
+    if (rc0) { delete mv; return rc0; }
+    hipStream_t s = ctx->stream;
+    PCR_HIP(hipMemcpyAsync(mv->B.d_views, views, sizeof(pcr_view) * nviews, hipMemcpyHostToDevice, s));
+    PCR_HIP(hipMemcpyAsync(mv->B.d_fbs, hf.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
+    PCR_HIP(hipMemcpyAsync(mv->B.d_out, ho.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
+    PCR_HIP(hipMemcpyAsync(mv->B.d_lod, hl.data(), sizeof(float) * nviews, hipMemcpyHostToDevice, s));
+    PCR_HIP(hipStreamSynchronize(s));
+    if (use_graph) {
+        PCR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = mv_enqueue(mv, s);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(s, &g);
+        if (rc != PCR_OK || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            mv_free(mv);
+            delete mv;
+            if (rc == PCR_OK) return hip_fail(ce, "hipStreamEndCapture", __FILE__, __LINE__);
+            return rc;
+        }
+        mv->graph = g;
+        PCR_HIP(hipGraphInstantiate(&mv->exec, g, nullptr, nullptr, 0));
+    }
+    *out = mv;
+    return PCR_OK;
+}
+
+int pcr_multiview_set_views(pcr_multiview* mv, const pcr_view* views, int nviews) {
+    PCR_CHECK(mv && views, "multiview_set_views: null argument");
+    PCR_CHECK(nviews == mv->nviews, "multiview_set_views: %d views, plan has %d", nviews, mv->nviews);
+    for (int v = 0; v < nviews; v++)
+        PCR_CHECK(views[v].width == mv->h_views[v].width && views[v].height == mv->h_views[v].height,
+                  "multiview_set_views: view %d changed size", v);
+    mv->h_views.assign(views, views + nviews);
+    PCR_HIP(hipMemcpyAsync(mv->B.d_views, mv->h_views.data(), sizeof(pcr_view) * nviews, hipMemcpyHostToDevice,
+                           mv->ctx->stream));
+    return PCR_OK;
+}
+
+int pcr_multiview_launch(pcr_multiview* mv) {
+    PCR_CHECK(mv, "multiview_launch: null plan");
+    if (mv->exec) {
+        PCR_HIP(hipGraphLaunch(mv->exec, mv->ctx->stream));
+        return PCR_OK;
+    }
+    return mv_enqueue(mv, mv->ctx->stream);
+}
+
+int pcr_multiview_destroy(pcr_multiview* mv) {
+    if (!mv) return PCR_OK;
+    (void)hipStreamSynchronize(mv->ctx->stream);
+    mv_free(mv);
+    delete mv;
+    return PCR_OK;
+}
+
+// Sweeps workgroup size x points per thread over the given views and keeps
+// the fastest configuration (median of `iters` launches, HIP events).
+int pcr_autotune(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nviews, pcr_fb* const* fbs,
+                 const pcr_render_params* params, int iters, pcr_tune* best, float* best_ms) {
+    PCR_CHECK(ctx && cloud && views && fbs && params && best && best_ms, "autotune: null argument");
+    PCR_CHECK(iters >= 1 && iters <= 1000, "autotune: iters %d out of range", iters);
+    const int blocks[4] = {64, 128, 256, 512};
+    const int ppts[3] = {40, 80, 160};
+    hipEvent_t e0, e1;
+    PCR_HIP(hipEventCreate(&e0));
+    PCR_HIP(hipEventCreate(&e1));
+    float best_t = 3.0e38f;
+    pcr_tune bt{256, 40, 0};
+    for (int bsz : blocks)
+        for (int ppt : ppts) {
+            pcr_tune t{bsz, ppt, 0};
+            pcr_multiview* mv = nullptr;
+            int rc = pcr_multiview_create(ctx, cloud, views, nviews, fbs, params, &t, 1, &mv);
+            if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+            std::vector<float> ts;
+            for (int it = 0; it < iters + 1; it++) {
+                PCR_HIP(hipEventRecord(e0, ctx->stream));
+                rc = pcr_multiview_launch(mv);
+                PCR_HIP(hipEventRecord(e1, ctx->stream));
+                PCR_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                PCR_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (it > 0) ts.push_back(ms);   // the first launch warms up
+            }
+            pcr_multiview_destroy(mv);
+            std::sort(ts.begin(), ts.end());
+            const float med = ts[ts.size() / 2];
+            if (med < best_t) { best_t = med; bt = t; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *best = bt;
+    *best_ms = best_t;
+    return PCR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,297 @@
+// pcr_octree.hip — Potree-style LOD octree in the batched plane layout.
+//
+// Stands in for the node layout the reference reads from Potree 2.0 files
+// (modules/compute/PotreeData.h:188-317: one 64-byte header per node, points
+// of a node contiguous, encoded relative to the node box as in
+// LasLoaderSparse.cpp:146-182) for synthetic clouds that have no Potree file.
+//
+// Construction (host):
+//   1. Morton keys (21 bits/axis over the bounding cube), stable LSD radix sort.
+//   2. Level l keeps, inside every level-l node (Morton prefix of 3l bits), the
+//      point of lowest seeded priority in each subsampling cell of side
+//      cube/(div * 2^l) (Morton prefix of 3(log2 div + l) bits) among the points
+//      not yet taken by a coarser level. A node whose remaining points fit in
+//      max_node_points keeps all of them and becomes a leaf.
+//   3. Nodes are emitted level by level in Morton order (parents precede
+//      children); a node with more than max_node_points points is split into
+//      chunks that share the node's header fields. first_point is 4-aligned.
+//   4. spacing = AABB diagonal / div / 2^level is the LOD spacing of the node.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+#include "pcr_generate.h"
+#include "pcr_internal.h"
+
+namespace pcr {
+
+static void radix_sort_pairs(std::vector<uint64_t>& keys, std::vector<uint32_t>& vals) {
+    const size_t n = keys.size();
+    std::vector<uint64_t> k2(n);
+    std::vector<uint32_t> v2(n);
+    for (int shift = 0; shift < 63; shift += 11) {
+        size_t cnt[2048] = {0};
+        for (size_t i = 0; i < n; i++) cnt[(keys[i] >> shift) & 2047]++;
+        size_t sum = 0;
+        for (int d = 0; d < 2048; d++) { const size_t c = cnt[d]; cnt[d] = sum; sum += c; }
+        for (size_t i = 0; i < n; i++) {
+            const size_t o = cnt[(keys[i] >> shift) & 2047]++;
+            k2[o] = keys[i];
+            v2[o] = vals[i];
+        }
+        keys.swap(k2);
+        vals.swap(v2);
+    }
+}
+
+static uint64_t spread3_host(uint32_t v) {
+    uint64_t x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+static uint32_t quant30(float x, float mn, double size) {
+    if (size == 0.0) return 0u;
+    const uint32_t v = (uint32_t)((((double)x - (double)mn) / size) * 1073741824.0);
+    return v < 1073741823u ? v : 1073741823u;
+}
+
+struct OctreeLayout {
+    std::vector<pcr_batch> hdr;
+    std::vector<uint32_t> lo, med, hi, col;
+    int depth = 0;
+    float mn[3], mx[3];
+};
+
+static int build_octree_layout(const float* xyz, const uint32_t* rgba, uint64_t n, const pcr_octree_opts* o,
+                               OctreeLayout& L) {
+    PCR_CHECK(xyz && rgba && o, "octree: null argument");
+    PCR_CHECK(n > 0 && n < (1ull << 32), "octree: point count out of range");
+    const uint32_t maxpts = o->max_node_points ? o->max_node_points : 20000;
+    const float div = o->root_spacing_div > 0 ? o->root_spacing_div : 128.0f;
+    const int gbits = (int)std::lround(std::log2((double)div));
+    PCR_CHECK(gbits >= 1 && gbits <= 12 && std::ldexp(1.0, gbits) == (double)div,
+              "octree: root_spacing_div %g must be a power of two in [2, 4096]", (double)div);
+    const int max_depth = std::min(o->max_depth > 0 ? o->max_depth : 12, 21 - gbits);
+
+    // 1. AABB, cube, Morton order
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint64_t i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++) {
+            const float v = xyz[3 * i + a];
+            mn[a] = v < mn[a] ? v : mn[a];
+            mx[a] = v > mx[a] ? v : mx[a];
+        }
+    float side = std::max(mx[0] - mn[0], std::max(mx[1] - mn[1], mx[2] - mn[2]));
+    const double scale = side > 0.0f ? 2097152.0 / (double)side : 0.0;
+    std::vector<uint64_t> keys(n);
+    std::vector<uint32_t> idx(n);
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t q[3];
+        for (int a = 0; a < 3; a++) {
+            const double d = ((double)xyz[3 * i + a] - (double)mn[a]) * scale;
+            q[a] = d >= 2097151.0 ? 2097151u : (uint32_t)d;
+        }
+        keys[i] = (spread3_host(q[0]) << 2) | (spread3_host(q[1]) << 1) | spread3_host(q[2]);
+        idx[i] = (uint32_t)i;
+    }
+    radix_sort_pairs(keys, idx);
+
+    // 2. level assignment
+    const uint64_t pkey = stream_key(o->seed, 5);
+    std::vector<int8_t> lev(n, -1);
+    uint64_t remaining = n;
+    int deepest = 0;
+    for (int l = 0; l <= max_depth && remaining; l++) {
+        const int node_shift = 63 - 3 * l;
+        const int cell_shift = 63 - 3 * (gbits + l);
+        size_t a = 0;
+        while (a < n) {
+            const uint64_t node = l == 0 ? 0 : keys[a] >> node_shift;
+            size_t b = a;
+            uint64_t un = 0;
+            while (b < n && (l == 0 || (keys[b] >> node_shift) == node)) { un += lev[b] < 0; b++; }
+            if (un) {
+                deepest = l;
+                if (un <= maxpts || l == max_depth) {
+                    for (size_t i = a; i < b; i++) if (lev[i] < 0) lev[i] = (int8_t)l;
+                    remaining -= un;
+                } else {
+                    size_t c = a;
+                    while (c < b) {
+                        const uint64_t cell = keys[c] >> cell_shift;
+                        size_t best = SIZE_MAX;
+                        uint64_t bp = ~0ull;
+                        size_t e = c;
+                        for (; e < b && (keys[e] >> cell_shift) == cell; e++) {
+                            if (lev[e] >= 0) continue;
+                            const uint64_t p = rnd(pkey, idx[e]);
+                            if (p < bp) { bp = p; best = e; }
+                        }
+                        if (best != SIZE_MAX) { lev[best] = (int8_t)l; remaining--; }
+                        c = e;
+                    }
+                }
+            }
+            a = b;
+        }
+    }
+
+    // 3. nodes, level by level in Morton order
+    const double diag = std::sqrt((double)(mx[0] - mn[0]) * (mx[0] - mn[0]) + (double)(mx[1] - mn[1]) * (mx[1] - mn[1]) +
+                                  (double)(mx[2] - mn[2]) * (mx[2] - mn[2]));
+    std::vector<pcr_batch> hdr;
+    std::vector<uint32_t> order;   // sorted position of each stored point, UINT32_MAX = padding
+    order.reserve(n + n / 1000 + 64);
+    std::vector<std::unordered_map<uint64_t, int32_t>> first_chunk(deepest + 1);
+    for (int l = 0; l <= deepest; l++) {
+        const int node_shift = 63 - 3 * l;
+        std::vector<uint32_t> pos;
+        for (uint64_t i = 0; i < n; i++) if (lev[i] == l) pos.push_back((uint32_t)i);
+        size_t a = 0;
+        while (a < pos.size()) {
+            const uint64_t node = l == 0 ? 0 : keys[pos[a]] >> node_shift;
+            size_t b = a;
+            float nmn[3] = {INFINITY, INFINITY, INFINITY}, nmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+            while (b < pos.size() && (l == 0 || (keys[pos[b]] >> node_shift) == node)) {
+                const float* p = xyz + 3ull * idx[pos[b]];
+                for (int k = 0; k < 3; k++) {
+                    nmn[k] = p[k] < nmn[k] ? p[k] : nmn[k];
+                    nmx[k] = p[k] > nmx[k] ? p[k] : nmx[k];
+                }
+                b++;
+            }
+            int32_t parent = -1;
+            if (l > 0) {
+                auto it = first_chunk[l - 1].find(node >> 3);
+                PCR_CHECK(it != first_chunk[l - 1].end(), "octree: node at level %d has no parent", l);
+                parent = it->second;
+            }
+            first_chunk[l][node] = (int32_t)hdr.size();
+            for (size_t c = a; c < b; c += maxpts) {
+                const size_t e = std::min(b, c + maxpts);
+                while (order.size() % 4) order.push_back(UINT32_MAX);
+                pcr_batch h;
+                memset(&h, 0, sizeof(h));
+                for (int k = 0; k < 3; k++) { h.min[k] = nmn[k]; h.max[k] = nmx[k]; }
+                h.num_points = (uint32_t)(e - c);
+                h.first_point = (uint32_t)order.size();
+                h.level = (uint32_t)l;
+                h.parent = parent;
+                h.spacing = (float)(diag / (double)div / std::ldexp(1.0, l));
+                hdr.push_back(h);
+                for (size_t q = c; q < e; q++) order.push_back(pos[q]);
+            }
+            a = b;
+        }
+    }
+    PCR_CHECK(order.size() < (1ull << 32), "octree: too many points after padding");
+
+    // 4. encode planes relative to each node box
+    const uint64_t np = order.size();
+    L.lo.assign(np, 0); L.med.assign(np, 0); L.hi.assign(np, 0); L.col.assign(np, 0);
+    uint32_t *lo = L.lo.data(), *med = L.med.data(), *hi = L.hi.data(), *col = L.col.data();
+    for (const pcr_batch& h : hdr) {
+        double size[3];
+        for (int k = 0; k < 3; k++) size[k] = (double)h.max[k] - (double)h.min[k];
+        for (uint32_t q = 0; q < h.num_points; q++) {
+            const uint64_t s = h.first_point + q;
+            const uint32_t src = idx[order[s]];
+            uint32_t qq[3];
+            for (int k = 0; k < 3; k++) qq[k] = quant30(xyz[3ull * src + k], h.min[k], size[k]);
+            lo[s] = ((qq[0] >> 20) & 1023u) | (((qq[1] >> 20) & 1023u) << 10) | (((qq[2] >> 20) & 1023u) << 20);
+            med[s] = ((qq[0] >> 10) & 1023u) | (((qq[1] >> 10) & 1023u) << 10) | (((qq[2] >> 10) & 1023u) << 20);
+            hi[s] = (qq[0] & 1023u) | ((qq[1] & 1023u) << 10) | ((qq[2] & 1023u) << 20);
+            col[s] = rgba[src];
+        }
+    }
+    L.hdr.swap(hdr);
+    L.depth = deepest;
+    for (int k = 0; k < 3; k++) { L.mn[k] = mn[k]; L.mx[k] = mx[k]; }
+    return PCR_OK;
+}
+
+int build_octree_host(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, uint64_t n, const pcr_octree_opts* o,
+                      pcr_cloud** out) {
+    PCR_CHECK(ctx && out, "octree: null argument");
+    OctreeLayout L;
+    int rc = build_octree_layout(xyz, rgba, n, o, L);
+    if (rc) return rc;
+    const uint64_t np = L.lo.size();
+    pcr_cloud* c = new pcr_cloud();
+    rc = cloud_alloc_batched(ctx, (uint32_t)L.hdr.size(), np, c);
+    if (rc) { delete c; return rc; }
+    c->batch_size = o->max_node_points ? o->max_node_points : 20000;
+    c->num_levels = L.depth + 1;
+    for (int k = 0; k < 3; k++) { c->aabb_min[k] = L.mn[k]; c->aabb_max[k] = L.mx[k]; }
+    PCR_HIP(hipMemcpy(c->d_batches, L.hdr.data(), sizeof(pcr_batch) * L.hdr.size(), hipMemcpyHostToDevice));
+    PCR_HIP(hipMemcpy(c->d_lo, L.lo.data(), np * 4, hipMemcpyHostToDevice));
+    PCR_HIP(hipMemcpy(c->d_med, L.med.data(), np * 4, hipMemcpyHostToDevice));
+    PCR_HIP(hipMemcpy(c->d_hi, L.hi.data(), np * 4, hipMemcpyHostToDevice));
+    PCR_HIP(hipMemcpy(c->d_rgba, L.col.data(), np * 4, hipMemcpyHostToDevice));
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    c->h_batches = L.hdr;
+    *out = c;
+    return PCR_OK;
+}
+
+}  // namespace pcr
+
+using namespace pcr;
+
+extern "C" {
+
+int pcr_cloud_build_octree_host(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, uint64_t n,
+                                const pcr_octree_opts* opts, pcr_cloud** out) {
+    return build_octree_host(ctx, xyz, rgba, n, opts, out);
+}
+
+int pcr_cloud_build_octree_device(pcr_ctx* ctx, const float* xyz_dev, const uint32_t* rgba_dev, uint64_t n,
+                                  const pcr_octree_opts* opts, pcr_cloud** out) {
+    PCR_CHECK(ctx && xyz_dev && rgba_dev && opts && out, "octree_device: null argument");
+    std::vector<float> xyz(3 * n);
+    std::vector<uint32_t> rgba(n);
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    PCR_HIP(hipMemcpy(xyz.data(), xyz_dev, 12 * n, hipMemcpyDeviceToHost));
+    PCR_HIP(hipMemcpy(rgba.data(), rgba_dev, 4 * n, hipMemcpyDeviceToHost));
+    return build_octree_host(ctx, xyz.data(), rgba.data(), n, opts, out);
+}
+
+int pcr_octree_layout_host(const float* xyz, const uint32_t* rgba, uint64_t n, const pcr_octree_opts* opts,
+                           pcr_batch* nodes, uint32_t* nnodes, uint32_t* lo, uint32_t* med, uint32_t* hi,
+                           uint32_t* rgba_out, uint64_t* nslots) {
+    PCR_CHECK(nnodes && nslots, "octree_layout_host: null size outputs");
+    OctreeLayout L;
+    int rc = build_octree_layout(xyz, rgba, n, opts, L);
+    if (rc) return rc;
+    const bool fill = nodes && lo && med && hi && rgba_out;
+    if (fill) {
+        PCR_CHECK(*nnodes >= L.hdr.size() && *nslots >= L.lo.size(),
+                  "octree_layout_host: arrays too small (%u nodes, %llu slots needed)", (unsigned)L.hdr.size(),
+                  (unsigned long long)L.lo.size());
+        memcpy(nodes, L.hdr.data(), sizeof(pcr_batch) * L.hdr.size());
+        memcpy(lo, L.lo.data(), 4 * L.lo.size());
+        memcpy(med, L.med.data(), 4 * L.med.size());
+        memcpy(hi, L.hi.data(), 4 * L.hi.size());
+        memcpy(rgba_out, L.col.data(), 4 * L.col.size());
+    }
+    *nnodes = (uint32_t)L.hdr.size();
+    *nslots = L.lo.size();
+    return PCR_OK;
+}
+
+int pcr_lod_selection(pcr_ctx* ctx, pcr_cloud* cloud, uint8_t* flags, uint32_t n) {
+    PCR_CHECK(ctx && cloud && flags, "lod_selection: null argument");
+    PCR_CHECK(n == cloud->num_batches, "lod_selection: %u flags for %u nodes", n, cloud->num_batches);
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    PCR_HIP(hipMemcpy(flags, cloud->d_levels, n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) flags[i] = flags[i] != 255;
+    return PCR_OK;
+}
+
+}  // extern "C"

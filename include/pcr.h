@@ -111,6 +111,10 @@ typedef struct pcr_render_params {
     int32_t  block_size;        /* raster workgroup size: 0 = default (256)        */
     int32_t  clear_before;      /* 1: clear the framebuffer before rasterising     */
     int32_t  time_stages;       /* 1: record HIP events around each stage          */
+    int32_t  tile_px;           /* LAS2/NODES: batches whose screen rectangle fits
+                                   in this many pixels merge in an LDS tile first
+                                   (0 = off, else 1024/2048/4096); same result   */
+    int32_t  pad[3];
 } pcr_render_params;
 
 typedef struct pcr_stats {
@@ -236,6 +240,13 @@ int pcr_cloud_build_octree_host(pcr_ctx* ctx, const float* xyz_host, const uint3
 /* Same, for points already on the device (sorting and level assignment on device). */
 int pcr_cloud_build_octree_device(pcr_ctx* ctx, const float* xyz_dev, const uint32_t* rgba_dev,
                                   uint64_t n, const pcr_octree_opts* opts, pcr_cloud** out);
+/* Host only (no device work): the same octree as a node array plus planes, for
+ * pcr_cloud_upload_batched. Call with null arrays to get *nnodes and *nslots
+ * (plane words incl. 4-alignment padding), then with arrays of those sizes. */
+int pcr_octree_layout_host(const float* xyz_host, const uint32_t* rgba_host, uint64_t n,
+                           const pcr_octree_opts* opts, pcr_batch* nodes_host, uint32_t* nnodes,
+                           uint32_t* lo_host, uint32_t* med_host, uint32_t* hi_host,
+                           uint32_t* rgba_out_host, uint64_t* nslots);
 /* The selected node set of the last PCR_MODE_NODES render: flags[i] = 1 if drawn. */
 int pcr_lod_selection(pcr_ctx* ctx, pcr_cloud* cloud, uint8_t* flags_host, uint32_t n);
 

@@ -1,0 +1,152 @@
+"""GPU: batched multi-view launch and the continuous-batching server.
+
+Each view of a multi-view launch, and each served request, must equal the
+single-view render of the same view bit for bit (and that render equals the
+oracle, checked here for one view per test and exhaustively in
+test_gpu_parity.py).
+"""
+import numpy as np
+import pytest
+
+import compute_rasterizer_amd as pcr
+from compute_rasterizer_amd import capi
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(256, 256), (640, 480), (1280, 720), (1920, 1080), (2560, 1440)]
+
+
+def views_for(sizes, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for w, h in sizes:
+        x, y = rng.uniform(300, 1700, 2)
+        ang = rng.uniform(0, 2 * np.pi)
+        eye = (x, y, rng.uniform(120, 400))
+        tgt = (x + 300 * np.cos(ang), y + 300 * np.sin(ang), 40.0)
+        out.append(pcr.look_at(eye, tgt, up=(0, 0, 1), near=1.0, far=5000.0, width=w, height=h))
+    return out
+
+
+@pytest.fixture(scope="module")
+def city(ctx):
+    cloud = ctx.generate(pcr.gen_cfg(pcr.SCENE_CITY, 2_000_000, seed=17, num_buildings=32))
+    yield cloud
+    cloud.close()
+
+
+@pytest.fixture(scope="module")
+def nodes(ctx):
+    xyz, rgba = orc.generate(pcr.gen_cfg(pcr.SCENE_TERRAIN, 300_000, seed=19))
+    cloud = ctx.octree_host(xyz, rgba, seed=4)
+    yield cloud
+    cloud.close()
+
+
+def single_view(ctx, cloud, view, prm):
+    fb = ctx.framebuffer(view.width, view.height)
+    ctx.render(cloud, view, prm, fb)
+    ctx.resolve(cloud, prm, fb)
+    out = fb.keys(), fb.rgba()
+    fb.close()
+    return out
+
+
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_multiview_equals_single_views(ctx, city, use_graph):
+    prm = pcr.params(pcr.MODE_LAS2)
+    views = views_for(SIZES * 2, seed=1)
+    fbs = [ctx.framebuffer(v.width, v.height) for v in views]
+    mv = pcr.MultiView(ctx, city, views, fbs, prm, use_graph=bool(use_graph))
+    for rnd in range(3):
+        if rnd:
+            views = views_for(SIZES * 2, seed=1 + rnd)
+            mv.set_views(views)
+        mv.launch()
+        ctx.synchronize()
+        # eager work between replays (a single-view render on the same stream)
+        for v, fb in zip(views, fbs):
+            keys, img = single_view(ctx, city, v, prm)
+            assert np.array_equal(fb.keys(), keys)
+            assert np.array_equal(fb.rgba(), img)
+    lay = city.download()
+    ref, _ = orc.render_batched(lay["batches"], lay["lo"], lay["med"], lay["hi"], views[1], 1)
+    assert np.array_equal(fbs[1].keys(), ref)
+    mv.close()
+
+
+def test_multiview_lod_nodes(ctx, nodes):
+    prm = pcr.params(pcr.MODE_NODES, lod_px=1.0)
+    views = views_for(SIZES, seed=7)
+    fbs = [ctx.framebuffer(v.width, v.height) for v in views]
+    mv = pcr.MultiView(ctx, nodes, views, fbs, prm)
+    mv.launch()
+    ctx.synchronize()
+    for v, fb in zip(views, fbs):
+        keys, img = single_view(ctx, nodes, v, prm)
+        assert np.array_equal(fb.keys(), keys)
+        assert np.array_equal(fb.rgba(), img)
+    mv.close()
+
+
+def test_autotune_picks_a_valid_config(ctx, city):
+    prm = pcr.params(pcr.MODE_LAS2)
+    views = views_for(SIZES, seed=3)
+    fbs = [ctx.framebuffer(v.width, v.height) for v in views]
+    best, ms = pcr.autotune(ctx, city, views, fbs, prm, iters=3)
+    assert best.block_size in (64, 128, 256, 512) and best.points_per_thread in (40, 80, 160)
+    assert ms > 0
+    mv = pcr.MultiView(ctx, city, views, fbs, prm, tune=best)
+    mv.launch()
+    ctx.synchronize()
+    keys, _ = single_view(ctx, city, views[2], prm)
+    assert np.array_equal(fbs[2].keys(), keys)
+    mv.close()
+
+
+def test_server_same_result_alone_and_in_mix(ctx, city, nodes):
+    big = 12 * 1920 * 1080
+    srv = pcr.Server(ctx, budget_bytes=3 * big, max_batch=4, num_streams=3)
+    c0, c1 = srv.add_cloud(city), srv.add_cloud(nodes)
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k, v in enumerate(views_for(SIZES[:4] * 4, seed=11)):
+        cid = int(rng.integers(0, 2))
+        mode = pcr.MODE_LAS2 if cid == c0 else pcr.MODE_NODES
+        reqs.append((cid, mode, v))
+    alone = []
+    for cid, mode, v in reqs:
+        t = srv.submit(cid, v, mode)
+        img, lat = srv.wait(t, v.width * v.height)
+        alone.append(img)
+        assert lat > 0
+    tickets = [srv.submit(cid, v, mode) for cid, mode, v in reqs]
+    mixed = [srv.wait(t, v.width * v.height)[0] for t, (_, _, v) in zip(tickets, reqs)]
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a, b)
+    # and both equal the plain single-view render
+    cid, mode, v = reqs[0]
+    _, img = single_view(ctx, city if cid == c0 else nodes, v, pcr.params(mode, lod_px=1.0))
+    assert np.array_equal(alone[0], img)
+    st = srv.stats()
+    assert st.completed == 2 * len(reqs)
+    assert st.fb_bytes_peak <= 3 * big and st.fb_bytes_in_use == 0
+    srv.close()
+
+
+def test_server_refuses_and_cancels(ctx, city):
+    srv = pcr.Server(ctx, budget_bytes=12 * 640 * 480 + 4096, max_batch=2, num_streams=1)
+    cid = srv.add_cloud(city)
+    with pytest.raises(pcr.PcrError) as e:
+        srv.submit(cid, views_for([(1920, 1080)], seed=1)[0])
+    assert e.value.code == capi.PCR_E_BUDGET
+    vs = views_for([(640, 480)] * 6, seed=2)
+    ts = [srv.submit(cid, v) for v in vs]
+    srv.cancel(ts[-1])
+    for t, v in zip(ts[:-1], vs[:-1]):
+        img, _ = srv.wait(t, v.width * v.height)
+        assert img.shape == (640 * 480,)
+    st = srv.stats()
+    assert st.refused == 1 and st.completed == 5 and st.fb_bytes_in_use == 0
+    srv.close()

@@ -169,3 +169,46 @@ def test_empty_frame(ctx, terrain):
     assert (fb.rgba() == prm.background).all()
     st = ctx.stats()
     assert st.batches_culled == st.batches_total
+
+
+@pytest.mark.parametrize("tile_px", [1024, 2048, 4096])
+def test_tiled_raster_matches_oracle(ctx, terrain, tile_px):
+    """LDS-tiled merge gives the same bits as the direct path, far and close views."""
+    cloud, lay = terrain
+    prm = pcr.params(pcr.MODE_LAS2, tile_px=tile_px)
+    views = terrain_views(n=3, seed=21) + [
+        pcr.look_at((1000, 1000, 100), (1000, 1010, 60), up=(0, 0, 1), near=0.5, far=4000, width=640, height=360)]
+    fb = ctx.framebuffer(640, 360)
+    for view in views:
+        ctx.render(cloud, view, prm, fb)
+        ref, _ = orc.render_batched(lay["batches"], lay["lo"], lay["med"], lay["hi"], view, 1)
+        keys = fb.keys()
+        assert np.array_equal(keys, ref), f"{(keys != ref).sum()} keys differ"
+
+
+@pytest.fixture(scope="module")
+def octree(ctx):
+    cfg = pcr.gen_cfg(pcr.SCENE_TERRAIN, 400_000, seed=13)
+    xyz, rgba = orc.generate(cfg)
+    cloud = ctx.octree_host(xyz, rgba, seed=3, max_node_points=20000)
+    lay = cloud.download()
+    yield cloud, lay
+    cloud.close()
+
+
+@pytest.mark.parametrize("rule", [pcr.LOD_SPACING, pcr.LOD_SIZE80])
+@pytest.mark.parametrize("tile_px", [0, 4096])
+def test_nodes_lod_matches_oracle(ctx, octree, rule, tile_px):
+    cloud, lay = octree
+    assert cloud.info().num_levels >= 2
+    fb = ctx.framebuffer(960, 540)
+    prm = pcr.params(pcr.MODE_NODES, lod_rule=rule, lod_px=1.0, tile_px=tile_px)
+    views = terrain_views(w=960, h=540, n=3, seed=31) + [
+        pcr.look_at((1000, -3000, 4000), (1000, 1000, 0), up=(0, 0, 1), near=1, far=20000, width=960, height=540)]
+    for view in views:
+        ctx.render(cloud, view, prm, fb)
+        ctx.resolve(cloud, prm, fb)
+        ref, flags = orc.render_nodes(lay["batches"], lay["lo"], lay["med"], lay["hi"], view, 1, rule, 1.0)
+        assert np.array_equal(ctx.lod_selection(cloud), flags)
+        assert np.array_equal(fb.keys(), ref)
+        assert np.array_equal(fb.rgba(), orc.resolve_index(ref, lay["rgba"], prm.background))
