@@ -120,6 +120,7 @@ SIGNATURES = {
     "pcr_synchronize": (C.c_int, [P]),
     "pcr_stream": (P, [P]),
     "pcr_view_look_at": (C.c_int, [dp, dp, dp, f64, f64, f64, dp, C.c_int, C.c_int, C.POINTER(View)]),
+    "pcr_view_from_matrices": (C.c_int, [dp, dp, dp, C.c_int, C.c_int, C.POINTER(View)]),
     "pcr_encode_batches_host": (C.c_int, [P, u64, u32, P, P, P, P]),
     "pcr_cloud_upload_points16": (C.c_int, [P, P, P, u64, PP]),
     "pcr_cloud_upload_batched": (C.c_int, [P, P, u32, P, P, P, P, u64, PP]),

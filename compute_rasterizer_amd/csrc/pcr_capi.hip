@@ -160,6 +160,35 @@ int pcr_view_look_at(const double eye[3], const double target[3], const double u
     return PCR_OK;
 }
 
+int pcr_view_from_matrices(const double view[16], const double proj[16], const double origin[3], int width,
+                           int height, pcr_view* out) {
+    PCR_CHECK(view && proj && origin && out, "view_from_matrices: null argument");
+    PCR_CHECK(width > 0 && height > 0, "view_from_matrices: bad size %dx%d", width, height);
+    double W[16] = {0}, VW[16], PVW[16];
+    W[0] = W[5] = W[10] = W[15] = 1.0;
+    W[12] = origin[0]; W[13] = origin[1]; W[14] = origin[2];
+    auto mul = [](const double* a, const double* b, double* r) {
+        for (int c = 0; c < 4; c++)
+            for (int rr = 0; rr < 4; rr++) {
+                double acc = 0.0;
+                for (int k = 0; k < 4; k++) acc += a[k * 4 + rr] * b[c * 4 + k];
+                r[c * 4 + rr] = acc;
+            }
+    };
+    mul(view, W, VW);
+    mul(proj, VW, PVW);
+    memset(out, 0, sizeof(*out));
+    for (int i = 0; i < 16; i++) {
+        out->transform[i] = (float)PVW[i];
+        out->transform_frustum[i] = (float)PVW[i];
+        out->view_world[i] = (float)VW[i];
+        out->proj[i] = (float)proj[i];
+    }
+    out->width = width;
+    out->height = height;
+    return PCR_OK;
+}
+
 int pcr_encode_batches_host(const float* xyz, uint64_t n, uint32_t batch_size, pcr_batch* batches,
                             uint32_t* lo, uint32_t* med, uint32_t* hi) {
     PCR_CHECK(xyz && batches && lo && med && hi, "encode_batches_host: null argument");

@@ -151,6 +151,13 @@ int pcr_view_look_at(const double eye[3], const double target[3], const double u
                      double fovy_deg, double near_plane, double far_plane,
                      const double origin[3], int width, int height, pcr_view* out);
 
+/* Build a pcr_view from the renderer's double view/projection matrices
+ * (Renderer.h:78-82 View::view/proj) and the cloud origin, exactly as
+ * ComputeLoopLas::render fills its per-file SSBO (compute_loop_las.h:166-199):
+ * transform = float(proj*view*translate(origin)) in double. */
+int pcr_view_from_matrices(const double view[16], const double proj[16], const double origin[3],
+                           int width, int height, pcr_view* out);
+
 /* Encode points into the adaptive-precision batch layout on the host
  * (LasLoaderSparse.cpp:37-212). Points are taken in the given order; batches
  * of `batch_size` points; planes lo/med/hi hold bits 29..20 / 19..10 / 9..0. */

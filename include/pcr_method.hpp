@@ -1,0 +1,140 @@
+// pcr_method.hpp — the reference's Method plugin interface, over libpcr's C ABI.
+//
+// Mirrors include/Method.h:10-24 (struct Method with name/description/group
+// and per-frame update/render), include/Runtime.h:17-50 (static method
+// registry, selection by name) and the headless part of Renderer::views
+// (include/Renderer.h:78-82). The four concrete methods stand in for
+// ComputeLoopLas (modules/compute_loop_las/compute_loop_las.h), ComputeLoopLas2
+// (compute_loop_las2.h), ComputeLoopLasHqs (compute_loop_las_hqs.h) and
+// ComputeLoopNodes (compute_loop_nodes.h): each render() rasterises the loaded
+// cloud for views[0] and resolves into that view's RGBA8 image. Errors throw
+// pcr_host::Error (the reference skipped silently, compute_loop_las.h:123,210).
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "pcr.h"
+
+namespace pcr_host {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+inline void check(int rc) {
+    if (rc != PCR_OK) throw Error(rc, pcr_last_error());
+}
+
+// Renderer::views[i]: double view/projection matrices (column-major) and the
+// resolved colour attachment, here a host RGBA8 array.
+struct View {
+    double view[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    double proj[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    int width = 0, height = 0;
+    std::vector<uint32_t> image;
+};
+
+struct Renderer {
+    pcr_ctx* ctx = nullptr;
+    std::vector<View> views;
+};
+
+struct Method {
+    std::string name = "no name";
+    std::string description = "";
+    std::string group = "no group";
+    virtual ~Method() = default;
+    virtual void update(Renderer* renderer) = 0;
+    virtual void render(Renderer* renderer) = 0;
+};
+
+struct Runtime {
+    inline static std::vector<Method*> methods;
+    inline static Method* selectedMethod = nullptr;
+    static void addMethod(Method* m) { methods.push_back(m); }
+    static void setSelectedMethod(const std::string& name) {
+        for (Method* m : methods)
+            if (m->name == name) selectedMethod = m;
+    }
+    static Method* getSelectedMethod() { return selectedMethod; }
+};
+
+// Shared body of the batched methods: one framebuffer per method, resized
+// when the view size changes (compute_loop_las.h:130-138), render + resolve
+// + readback of the image each frame.
+class BatchedMethod : public Method {
+  public:
+    BatchedMethod(pcr_ctx* ctx, pcr_cloud* cloud, const double origin[3], int mode, int fb_kind)
+        : ctx_(ctx), cloud_(cloud), mode_(mode), fb_kind_(fb_kind) {
+        for (int i = 0; i < 3; i++) origin_[i] = origin ? origin[i] : 0.0;
+        params_ = pcr_render_params{};
+        params_.mode = mode;
+        params_.frustum_culling = 1;
+        params_.lod_rule = PCR_LOD_SPACING;
+        params_.lod_px = 1.0f;
+        params_.background = PCR_BG_LAS;
+        params_.clear_before = 1;
+    }
+    ~BatchedMethod() override { pcr_fb_destroy(fb_); }
+    void update(Renderer*) override {}
+    void render(Renderer* r) override {
+        if (!cloud_ || r->views.empty()) return;   // nothing loaded (compute_loop_las.h:123)
+        View& v = r->views[0];
+        if (!fb_ || fb_w_ != v.width || fb_h_ != v.height) {
+            pcr_fb_destroy(fb_);
+            fb_ = nullptr;
+            check(pcr_fb_create(ctx_, v.width, v.height, fb_kind_, &fb_));
+            fb_w_ = v.width;
+            fb_h_ = v.height;
+        }
+        pcr_view pv;
+        check(pcr_view_from_matrices(v.view, v.proj, origin_, v.width, v.height, &pv));
+        check(pcr_render(ctx_, cloud_, &pv, &params_, fb_));
+        check(pcr_resolve(ctx_, cloud_, &params_, fb_));
+        v.image.resize((size_t)v.width * v.height);
+        check(pcr_fb_readback_rgba(ctx_, fb_, v.image.data()));
+    }
+    pcr_render_params& params() { return params_; }
+    pcr_fb* framebuffer() { return fb_; }
+
+  protected:
+    pcr_ctx* ctx_;
+    pcr_cloud* cloud_;
+    double origin_[3];
+    int mode_, fb_kind_;
+    pcr_render_params params_;
+    pcr_fb* fb_ = nullptr;
+    int fb_w_ = 0, fb_h_ = 0;
+};
+
+struct ComputeLoopLas : BatchedMethod {       // compute_loop_las.h:80-92
+    ComputeLoopLas(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : BatchedMethod(c, cl, o, PCR_MODE_LAS, PCR_FB_KEY64) {
+        name = "loop_las";
+        description = "one workgroup per batch, frustum cull, adaptive 10/20/30-bit precision";
+        group = "10-10-10 bit encoded";
+    }
+};
+struct ComputeLoopLas2 : BatchedMethod {      // compute_loop_las2.h:80-94
+    ComputeLoopLas2(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : BatchedMethod(c, cl, o, PCR_MODE_LAS2, PCR_FB_KEY64) {
+        name = "loop_las_prefetch";
+        description = "as loop_las, four consecutive points per thread with 128-bit loads";
+        group = "10-10-10 bit encoded";
+    }
+};
+struct ComputeLoopLasHqs : BatchedMethod {    // compute_loop_las_hqs.h:86-95
+    ComputeLoopLasHqs(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : BatchedMethod(c, cl, o, PCR_MODE_HQS, PCR_FB_HQS) {
+        name = "loop_las_hqs";
+        description = "high-quality shading: depth pass, then averaging of points within 1% of the depth";
+        group = "10-10-10 bit encoded";
+    }
+};
+struct ComputeLoopNodes : BatchedMethod {     // compute_loop_nodes.h:74-84
+    ComputeLoopNodes(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : BatchedMethod(c, cl, o, PCR_MODE_NODES, PCR_FB_KEY64) {
+        name = "loop_nodes";
+        description = "LOD octree nodes: frustum and screen-space error cut";
+        group = "render Potree nodes";
+    }
+};
+
+}  // namespace pcr_host
