@@ -45,6 +45,70 @@ def flythrough(n, width, height, seed=42, extent=2000.0, altitude=250.0):
     return views
 
 
+MV_SIZES = [(256, 256), (640, 480), (1280, 720), (1920, 1080), (2560, 1440)]
+
+
+def multiview_views(n, seed=7, extent=2000.0):
+    """Seeded cameras over the city at uneven resolutions."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(n):
+        w, h = MV_SIZES[int(rng.integers(0, len(MV_SIZES)))]
+        x, y = rng.uniform(0.15 * extent, 0.85 * extent, 2)
+        a = rng.uniform(0, 2 * math.pi)
+        eye = (x, y, rng.uniform(150.0, 400.0))
+        tgt = (x + 500 * math.cos(a), y + 500 * math.sin(a), rng.uniform(0.0, 80.0))
+        views.append(pcr.look_at(eye, tgt, up=(0, 0, 1), fovy=60.0, near=1.0, far=5000.0, width=w, height=h))
+    return views
+
+
+def multiview_leg(ctx, cloud, n_points, nviews, frames, tile_px):
+    """Plain per-view frames vs one autotuned persistent launch for all views."""
+    views = multiview_views(nviews)
+    fbs = [ctx.framebuffer(v.width, v.height) for v in views]
+    prm = pcr.params(pcr.MODE_LAS2, tile_px=tile_px)
+
+    def plain_frame():
+        for v, fb in zip(views, fbs):
+            ctx.render(cloud, v, prm, fb)
+            ctx.resolve(cloud, prm, fb)
+
+    plain_frame()
+    ctx.synchronize()
+    ref = [(fbs[k].keys(), fbs[k].rgba()) for k in range(nviews)]
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        plain_frame()
+    ctx.synchronize()
+    plain_s = (time.perf_counter() - t0) / frames
+
+    t = time.perf_counter()
+    best, best_ms = pcr.autotune(ctx, cloud, views, fbs, prm, iters=3)
+    tune_s = time.perf_counter() - t
+    mv = pcr.MultiView(ctx, cloud, views, fbs, prm, tune=best)
+    mv.launch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        mv.launch()
+    ctx.synchronize()
+    mv_s = (time.perf_counter() - t0) / frames
+    exact = all(np.array_equal(fbs[k].keys(), ref[k][0]) and np.array_equal(fbs[k].rgba(), ref[k][1])
+                for k in range(nviews))
+    mv.close()
+    for fb in fbs:
+        fb.close()
+    pix = sum(v.width * v.height for v in views)
+    return {"views": nviews, "megapixels": round(pix / 1e6, 2), "frames": frames,
+            "plain_ms_per_frame": round(plain_s * 1e3, 3), "batched_ms_per_frame": round(mv_s * 1e3, 3),
+            "plain_Gpoints_s": round(n_points * nviews / plain_s / 1e9, 1),
+            "batched_Gpoints_s": round(n_points * nviews / mv_s / 1e9, 1),
+            "speedup": round(plain_s / mv_s, 3),
+            "tune": {"block_size": best.block_size, "points_per_thread": best.points_per_thread,
+                     "tile_px": best.tile_px, "median_ms": round(best_ms, 3), "sweep_s": round(tune_s, 2)},
+            "bit_exact_vs_plain": bool(exact)}
+
+
 def cpu_baseline(ctx, cloud, views, gpu_fb, prm, frames, n_points):
     """Oracle (multithreaded C) on the host cores for `frames` full frames;
     the GPU framebuffers of the same views are compared bit for bit."""
@@ -78,7 +142,10 @@ def main():
     ap.add_argument("--views", type=int, default=64)
     ap.add_argument("--mode", default="las2", choices=["las", "las2"])
     ap.add_argument("--cpu-frames", type=int, default=2, help="0 skips the CPU baseline")
-    ap.add_argument("--tile-px", type=int, default=0, help="LDS tile size of the tiled raster (0 = direct)")
+    ap.add_argument("--tile-px", type=int, default=2048, help="LDS tile size of the tiled raster (0 = direct)")
+    ap.add_argument("--block", type=int, default=0, help="raster workgroup size (0 = default)")
+    ap.add_argument("--mv-views", type=int, default=48, help="views of the batched multi-view leg (0 skips it)")
+    ap.add_argument("--mv-frames", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 device copies of 1 GiB (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -111,7 +178,8 @@ def main():
     W, H = args.width, args.height
     views = flythrough(args.views, W, H)
     mode = pcr.MODE_LAS2 if args.mode == "las2" else pcr.MODE_LAS
-    prm = pcr.params(mode, time_stages=1, tile_px=args.tile_px)
+    prm = pcr.params(mode, time_stages=1, tile_px=args.tile_px if mode == pcr.MODE_LAS2 else 0,
+                     block_size=args.block)
     fb = ctx.framebuffer(W, H)
 
     # per-view algorithmic bytes of the raster kernel: headers of drawn batches,
@@ -177,10 +245,16 @@ def main():
         "frame_algorithmic_GBs": round(frame_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_raster_tiled<256,%d>" % args.tile_px if args.tile_px else "k_raster_batched<256,4>",
+                     "kernel": ("k_raster_tiled<%d,%d>" % (args.block if args.block in (128, 256) else 256, args.tile_px)
+                                if args.tile_px and mode == pcr.MODE_LAS2 else
+                                "k_raster_batched<%d,%d>" % (args.block or (256 if mode == pcr.MODE_LAS2 else 128),
+                                                             4 if mode == pcr.MODE_LAS2 else 1)),
                      "bytes_per_launch_mean": int(np.mean([raster_bytes[i] for i in idx]))},
         "generate_s": round(gen_s, 2),
     }
+    if args.mv_views > 0:
+        out["multiview"] = multiview_leg(ctx, cloud, info.num_points, args.mv_views, args.mv_frames, args.tile_px)
+        print(f"[bench] multiview {out['multiview']}", file=sys.stderr, flush=True)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(ctx, cloud, views, fb, prm, args.cpu_frames, info.num_points)
     print(json.dumps(out), flush=True)

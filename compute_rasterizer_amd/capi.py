@@ -92,7 +92,8 @@ class OctreeOpts(C.Structure):
 
 
 class Tune(C.Structure):
-    _fields_ = [("block_size", C.c_int32), ("points_per_thread", C.c_int32), ("persistent_groups", C.c_int32)]
+    _fields_ = [("block_size", C.c_int32), ("points_per_thread", C.c_int32), ("persistent_groups", C.c_int32),
+                ("tile_px", C.c_int32)]
 
 
 class Request(C.Structure):

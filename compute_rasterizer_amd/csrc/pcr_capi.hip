@@ -455,7 +455,7 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
         case PCR_MODE_LAS: rc = launch_raster_batched(s, cloud, *view, 1, block ? block : 128, ctx->d_state, fb->d_keys); break;
         case PCR_MODE_LAS2:
         case PCR_MODE_NODES:
-            rc = params->tile_px ? launch_raster_tiled(s, cloud, *view, params->tile_px, fb->d_keys)
+            rc = params->tile_px ? launch_raster_tiled(s, cloud, *view, params->tile_px, block, ctx->d_state, fb->d_keys)
                                  : launch_raster_batched(s, cloud, *view, 4, block ? block : 256, ctx->d_state, fb->d_keys);
             break;
         case PCR_MODE_HQS: rc = launch_hqs_batched(s, cloud, *view, ctx->d_state, fb); break;

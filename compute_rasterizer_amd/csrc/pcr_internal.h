@@ -88,14 +88,18 @@ int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pc
                   FrameState* st, float first_threshold, bool lod);
 int launch_raster_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int vec, int block,
                           FrameState* st, unsigned long long* fb);
-int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, unsigned long long* fb);
+int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block,
+                        FrameState* st, unsigned long long* fb);
 int launch_raster_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb);
 int launch_hqs_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb);
 int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pcr_fb* fb);
 int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, pcr_fb* fb);
 
 // multi-view frame (pcr_multiview.hip), shared with the server
-struct MVState;
+struct MVState {
+    uint32_t list_count;   // batches drawn by at least one view
+    uint32_t queue_head;   // work items taken by the persistent raster
+};
 struct MVBuffers {
     pcr_view* d_views = nullptr;
     unsigned long long** d_fbs = nullptr;
@@ -111,7 +115,9 @@ struct MVBuffers {
 int mv_alloc(MVBuffers& B, int nviews, uint64_t nb);
 void mv_release(MVBuffers& B);
 int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, uint64_t max_pix,
-                   const pcr_render_params& prm, bool lod, const pcr_tune& t, int groups);
+                   const pcr_render_params& prm, bool lod, const pcr_tune& t, int cu_count);
+int launch_mv_raster_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
+                           int cu_count);
 
 // layout (pcr_layout.hip)
 int generate_points(pcr_ctx* ctx, const pcr_gen_cfg* cfg, float* xyz, uint32_t* rgba);

@@ -247,7 +247,15 @@ __device__ __forceinline__ void tile_point(const Decode& d, uint32_t lo, uint32_
     else fb_min(fb, (uint32_t)ix + (uint32_t)iy * (uint32_t)v.width, key);
 }
 
-template <int BLOCK, int LC, bool TILE>
+// NT: non-temporal plane loads (a single view streams each plane once);
+// multi-view passes re-read a batch once per view and keep it cached.
+template <bool NT>
+__device__ __forceinline__ u32x4 ld_plane(const u32x4* p) {
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
+template <int BLOCK, int LC, bool TILE, bool NT>
 __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __restrict__ lo,
                                           const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                           uint32_t first, uint32_t np, const pcr_view& v,
@@ -258,9 +266,9 @@ __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __res
     const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + first);
     const uint32_t nq = (np + 3u) >> 2;
     for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
-        const u32x4 a = ld_stream(lo4 + q);
-        const u32x4 m = LC <= 1 ? ld_stream(med4 + q) : u32x4{0, 0, 0, 0};
-        const u32x4 c = LC == 0 ? ld_stream(hi4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 a = ld_plane<NT>(lo4 + q);
+        const u32x4 m = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
         const uint32_t base = q * 4u;
         tile_point<LC, TILE>(d, a.x, m.x, c.x, base + 0, np, first, v, fb, tile, x0, y0, tw, th);
         tile_point<LC, TILE>(d, a.y, m.y, c.y, base + 1, np, first, v, fb, tile, x0, y0, tw, th);
@@ -269,27 +277,25 @@ __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __res
     }
 }
 
-template <int BLOCK, bool TILE>
+template <int BLOCK, bool TILE, bool NT>
 __device__ __forceinline__ void tile_dispatch(int level, const Decode& d, const uint32_t* __restrict__ lo,
                                               const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                               const pcr_batch& h, const pcr_view& v, unsigned long long* __restrict__ fb,
                                               unsigned long long* tile, int x0, int y0, int tw, int th) {
-    if (level == 0) tile_loop<BLOCK, 0, TILE>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
-    else if (level == 1) tile_loop<BLOCK, 1, TILE>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
-    else tile_loop<BLOCK, 2, TILE>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
+    if (level == 0)
+        tile_loop<BLOCK, 0, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
+    else if (level == 1)
+        tile_loop<BLOCK, 1, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
+    else
+        tile_loop<BLOCK, 2, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
 }
 
-template <int BLOCK, int TILE_PIX>
-__global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restrict__ b,
-                                                        const uint8_t* __restrict__ levels,
-                                                        const uint32_t* __restrict__ lo,
-                                                        const uint32_t* __restrict__ med,
-                                                        const uint32_t* __restrict__ hi, pcr_view v,
-                                                        unsigned long long* __restrict__ fb) {
-    __shared__ unsigned long long tile[TILE_PIX];
-    const uint32_t bi = blockIdx.x;
-    const int level = levels[bi];
-    if (level == kCulled) return;
+// One batch of the tiled raster; `bi` and `level` are workgroup-uniform.
+template <int BLOCK, int TILE_PIX, bool NT>
+__device__ __forceinline__ void tiled_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
+                                            const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
+                                            const uint32_t* __restrict__ hi, const pcr_view& v,
+                                            unsigned long long* __restrict__ fb, unsigned long long* tile) {
     const pcr_batch h = b[bi];
     const Decode d = make_decode(level, h.min, h.max);
     int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
@@ -302,7 +308,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restr
         const int n = tw * th;
         for (int i = threadIdx.x; i < n; i += BLOCK) tile[i] = PCR_FB_CLEAR;
         __syncthreads();
-        tile_dispatch<BLOCK, true>(level, d, lo, med, hi, h, v, fb, tile, x0, y0, tw, th);
+        tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, v, fb, tile, x0, y0, tw, th);
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += BLOCK) {
             const unsigned long long k = tile[i];
@@ -312,25 +318,144 @@ __global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restr
             }
         }
     } else {
-        tile_dispatch<BLOCK, false>(level, d, lo, med, hi, h, v, fb, nullptr, 0, 0, 0, 0);
+        tile_dispatch<BLOCK, false, NT>(level, d, lo, med, hi, h, v, fb, nullptr, 0, 0, 0, 0);
     }
 }
 
-int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, unsigned long long* fb) {
+// Persistent workgroups take drawn batches from the select stage's compacted
+// list through an atomic queue, so no workgroup is launched for a culled
+// batch. The loop ends when the queue passes list_count, which every
+// workgroup reaches.
+template <int BLOCK, int TILE_PIX>
+__global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restrict__ b,
+                                                        const uint8_t* __restrict__ levels,
+                                                        const uint32_t* __restrict__ list,
+                                                        FrameState* __restrict__ st,
+                                                        const uint32_t* __restrict__ lo,
+                                                        const uint32_t* __restrict__ med,
+                                                        const uint32_t* __restrict__ hi, pcr_view v,
+                                                        unsigned long long* __restrict__ fb) {
+    __shared__ unsigned long long tile[TILE_PIX];
+    __shared__ uint32_t s_item;
+    const uint32_t count = st->list_count;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&st->queue_head, 1u);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= count) break;
+        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[item]);
+        const int level = __builtin_amdgcn_readfirstlane(levels[bi]);
+        tiled_batch<BLOCK, TILE_PIX, true>(bi, level, b, lo, med, hi, v, fb, tile);
+        __syncthreads();   // s_item and the tile are reused by the next item
+    }
+}
+
+template <int BLOCK, int TILE_PIX>
+static void tiled_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, unsigned long long* fb) {
+    static const int occ = [] {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_raster_tiled<BLOCK, TILE_PIX>, BLOCK, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            o = 0;
+        }
+        return o > 0 ? o : 4;
+    }();
+    uint64_t grid = (uint64_t)c->ctx->cu_count * (uint64_t)occ;
+    if (grid > c->num_batches) grid = c->num_batches;
+    k_raster_tiled<BLOCK, TILE_PIX><<<(int)grid, BLOCK, 0, s>>>(c->d_batches, c->d_levels, c->d_list, st, c->d_lo,
+                                                               c->d_med, c->d_hi, v, fb);
+}
+
+int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block,
+                        FrameState* st, unsigned long long* fb) {
     if (c->num_batches == 0) return PCR_OK;
-    switch (tile_px) {
-        case 1024:
-            k_raster_tiled<256, 1024><<<c->num_batches, 256, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
-                                                                     c->d_hi, v, fb);
-            break;
-        case 2048:
-            k_raster_tiled<256, 2048><<<c->num_batches, 256, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
-                                                                     c->d_hi, v, fb);
-            break;
-        default:
-            k_raster_tiled<256, 4096><<<c->num_batches, 256, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
-                                                                     c->d_hi, v, fb);
-            break;
+    if (block == 128) {
+        if (tile_px == 1024) tiled_t<128, 1024>(s, c, v, st, fb);
+        else if (tile_px == 4096) tiled_t<128, 4096>(s, c, v, st, fb);
+        else tiled_t<128, 2048>(s, c, v, st, fb);
+    } else {
+        if (tile_px == 1024) tiled_t<256, 1024>(s, c, v, st, fb);
+        else if (tile_px == 4096) tiled_t<256, 4096>(s, c, v, st, fb);
+        else tiled_t<256, 2048>(s, c, v, st, fb);
+    }
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+// Multi-view tiled raster: persistent workgroups take batches kept by any
+// view; for each view that draws the batch the workgroup runs the tiled
+// batch pass with that view's level. After the first view the batch's planes
+// come from cache, so HBM sees each plane once per frame.
+template <int BLOCK, int TILE_PIX>
+__global__ __launch_bounds__(BLOCK) void k_mv_raster_tiled(const pcr_batch* __restrict__ b, uint32_t nb,
+                                                           const uint8_t* __restrict__ levels,
+                                                           const uint32_t* __restrict__ list,
+                                                           MVState* __restrict__ st, const uint32_t* __restrict__ lo,
+                                                           const uint32_t* __restrict__ med,
+                                                           const uint32_t* __restrict__ hi,
+                                                           const pcr_view* __restrict__ views, int nviews,
+                                                           unsigned long long* const* __restrict__ fbs) {
+    __shared__ unsigned long long tile[TILE_PIX];
+    __shared__ uint32_t s_item, s_nv;
+    __shared__ uint8_t s_vid[64], s_lv[64];
+    const uint32_t count = st->list_count;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&st->queue_head, 1u);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= count) break;
+        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[item]);
+        if (threadIdx.x < 64) {   // compact the views drawing this batch (wave 0)
+            const int v = (int)threadIdx.x;
+            const int lv = v < nviews ? levels[(uint64_t)v * nb + bi] : kCulled;
+            const unsigned long long m = __ballot(lv != kCulled);
+            if (lv != kCulled) {
+                const uint32_t pos = (uint32_t)__popcll(m & ((1ull << v) - 1ull));
+                s_vid[pos] = (uint8_t)v;
+                s_lv[pos] = (uint8_t)lv;
+            }
+            if (v == 0) s_nv = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        const uint32_t nv = __builtin_amdgcn_readfirstlane(s_nv);
+        for (uint32_t k = 0; k < nv; k++) {
+            const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
+            const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
+            tiled_batch<BLOCK, TILE_PIX, false>(bi, lv, b, lo, med, hi, views[vi], fbs[vi], tile);
+            __syncthreads();   // the tile is reused by the next view
+        }
+    }
+}
+
+template <int BLOCK, int TILE_PIX>
+static void mv_tiled_t(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int cu_count) {
+    static const int occ = [] {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_mv_raster_tiled<BLOCK, TILE_PIX>, BLOCK, 0) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            o = 0;
+        }
+        return o > 0 ? o : 4;
+    }();
+    uint64_t grid = (uint64_t)cu_count * (uint64_t)occ;
+    if (grid > c->num_batches) grid = c->num_batches;
+    if (grid == 0) grid = 1;
+    k_mv_raster_tiled<BLOCK, TILE_PIX><<<(int)grid, BLOCK, 0, s>>>(c->d_batches, c->num_batches, B.d_levels, B.d_list,
+                                                                  B.d_state, c->d_lo, c->d_med, c->d_hi, B.d_views,
+                                                                  nviews, B.d_fbs);
+}
+
+int launch_mv_raster_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
+                           int cu_count) {
+    if (block == 128) {
+        if (tile_px == 1024) mv_tiled_t<128, 1024>(s, c, B, nviews, cu_count);
+        else if (tile_px == 4096) mv_tiled_t<128, 4096>(s, c, B, nviews, cu_count);
+        else mv_tiled_t<128, 2048>(s, c, B, nviews, cu_count);
+    } else {
+        if (tile_px == 1024) mv_tiled_t<256, 1024>(s, c, B, nviews, cu_count);
+        else if (tile_px == 4096) mv_tiled_t<256, 4096>(s, c, B, nviews, cu_count);
+        else mv_tiled_t<256, 2048>(s, c, B, nviews, cu_count);
     }
     PCR_HIP(hipGetLastError());
     return PCR_OK;

@@ -25,11 +25,6 @@ namespace pcr {
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kMaxViews = 64;
 
-struct MVState {
-    uint32_t list_count;
-    uint32_t queue_head;
-};
-
 __global__ void k_mv_reset(MVState* st) {
     if (threadIdx.x == 0) { st->list_count = 0; st->queue_head = 0; }
 }
@@ -221,7 +216,8 @@ void mv_release(MVBuffers& B) {
 
 // Queues one multi-view frame: clear, reset, select, raster, resolve.
 int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, uint64_t max_pix,
-                   const pcr_render_params& prm, bool lod, const pcr_tune& t, int groups) {
+                   const pcr_render_params& prm, bool lod, const pcr_tune& t, int cu_count) {
+    const int groups = t.persistent_groups > 0 ? t.persistent_groups : cu_count * (2048 / t.block_size);
     uint32_t maxnp = 4;
     for (const pcr_batch& h : c->h_batches) maxnp = maxnp > h.num_points ? maxnp : h.num_points;
     const uint32_t chunk_pts = (uint32_t)(t.block_size * t.points_per_thread);
@@ -237,7 +233,10 @@ int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nv
     k_mv_raster<BS><<<groups, BS, 0, s>>>(c->d_batches, c->num_batches, B.d_levels, B.d_best, B.d_list,          \
                                           B.d_state, c->d_lo, c->d_med, c->d_hi, B.d_views, nviews, B.d_fbs,    \
                                           chunk_pts, cpb)
-    if (t.block_size == 64) MV_RASTER(64);
+    if (t.tile_px) {
+        int rc = launch_mv_raster_tiled(s, c, B, nviews, t.tile_px, t.block_size, cu_count);
+        if (rc) return rc;
+    } else if (t.block_size == 64) MV_RASTER(64);
     else if (t.block_size == 128) MV_RASTER(128);
     else if (t.block_size == 512) MV_RASTER(512);
     else MV_RASTER(256);
@@ -261,7 +260,6 @@ struct pcr_multiview {
     std::vector<pcr_view> h_views;
     MVBuffers B;
     uint64_t max_pix = 0;
-    int groups = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -269,7 +267,8 @@ struct pcr_multiview {
 namespace pcr {
 
 static int mv_enqueue(pcr_multiview* mv, hipStream_t s) {
-    return mv_enqueue_raw(s, mv->cloud, mv->B, mv->nviews, mv->max_pix, mv->prm, mv->lod, mv->tune, mv->groups);
+    return mv_enqueue_raw(s, mv->cloud, mv->B, mv->nviews, mv->max_pix, mv->prm, mv->lod, mv->tune,
+                          mv->ctx->cu_count);
 }
 
 static void mv_free(pcr_multiview* mv) {
@@ -295,18 +294,20 @@ int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, 
                   "multiview_create: view %d is %dx%d but its framebuffer is %dx%d", v, views[v].width,
                   views[v].height, fbs[v]->width, fbs[v]->height);
     }
-    pcr_tune t = tune ? *tune : pcr_tune{256, 40, 0};
+    pcr_tune t = tune ? *tune : pcr_tune{128, 40, 0, 2048};
     if (t.block_size == 0) t.block_size = 256;
     if (t.points_per_thread == 0) t.points_per_thread = 40;
     PCR_CHECK(t.block_size == 64 || t.block_size == 128 || t.block_size == 256 || t.block_size == 512,
               "multiview_create: block_size %d not in {64,128,256,512}", t.block_size);
     PCR_CHECK(t.points_per_thread > 0 && t.points_per_thread % 4 == 0 && t.points_per_thread <= 1024,
               "multiview_create: points_per_thread %d must be a positive multiple of 4", t.points_per_thread);
+    PCR_CHECK(t.tile_px == 0 || ((t.tile_px == 1024 || t.tile_px == 2048 || t.tile_px == 4096) &&
+                                 (t.block_size == 128 || t.block_size == 256)),
+              "multiview_create: tile_px %d needs 1024/2048/4096 with block 128 or 256", t.tile_px);
     pcr_multiview* mv = new pcr_multiview();
     mv->ctx = ctx; mv->cloud = cloud; mv->nviews = nviews; mv->prm = *params; mv->tune = t;
     mv->lod = params->mode == PCR_MODE_NODES;
     mv->h_views.assign(views, views + nviews);
-    mv->groups = t.persistent_groups > 0 ? t.persistent_groups : ctx->cu_count * (2048 / t.block_size);
     std::vector<unsigned long long*> hf(nviews);
     std::vector<uint32_t*> ho(nviews);
     std::vector<float> hl(nviews, params->lod_px);
@@ -371,40 +372,45 @@ int pcr_multiview_destroy(pcr_multiview* mv) {
     return PCR_OK;
 }
 
-// Sweeps workgroup size x points per thread over the given views and keeps
-// the fastest configuration (median of `iters` launches, HIP events).
+// Sweeps the untiled kernel over workgroup size {64,128,256,512} x points per
+// thread {40,80,160} and the LDS-tiled kernel over {128,256} x tile
+// {1024,2048,4096} px for the given views; keeps the fastest configuration
+// (median of `iters` graph launches, HIP events).
 int pcr_autotune(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nviews, pcr_fb* const* fbs,
                  const pcr_render_params* params, int iters, pcr_tune* best, float* best_ms) {
     PCR_CHECK(ctx && cloud && views && fbs && params && best && best_ms, "autotune: null argument");
     PCR_CHECK(iters >= 1 && iters <= 1000, "autotune: iters %d out of range", iters);
-    const int blocks[4] = {64, 128, 256, 512};
-    const int ppts[3] = {40, 80, 160};
+    std::vector<pcr_tune> space;
+    for (int bsz : {64, 128, 256, 512})
+        for (int ppt : {40, 80, 160}) space.push_back(pcr_tune{bsz, ppt, 0, 0});
+    for (int bsz : {128, 256})
+        for (int tile : {1024, 2048, 4096}) space.push_back(pcr_tune{bsz, 40, 0, tile});
     hipEvent_t e0, e1;
     PCR_HIP(hipEventCreate(&e0));
     PCR_HIP(hipEventCreate(&e1));
     float best_t = 3.0e38f;
-    pcr_tune bt{256, 40, 0};
-    for (int bsz : blocks)
-        for (int ppt : ppts) {
-            pcr_tune t{bsz, ppt, 0};
-            pcr_multiview* mv = nullptr;
-            int rc = pcr_multiview_create(ctx, cloud, views, nviews, fbs, params, &t, 1, &mv);
-            if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
-            std::vector<float> ts;
-            for (int it = 0; it < iters + 1; it++) {
-                PCR_HIP(hipEventRecord(e0, ctx->stream));
-                rc = pcr_multiview_launch(mv);
-                PCR_HIP(hipEventRecord(e1, ctx->stream));
-                PCR_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                PCR_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (it > 0) ts.push_back(ms);   // the first launch warms up
-            }
-            pcr_multiview_destroy(mv);
-            std::sort(ts.begin(), ts.end());
-            const float med = ts[ts.size() / 2];
-            if (med < best_t) { best_t = med; bt = t; }
+    pcr_tune bt = space[0];
+    for (const pcr_tune& cand : space) {
+        pcr_tune t = cand;
+        pcr_multiview* mv = nullptr;
+        int rc = pcr_multiview_create(ctx, cloud, views, nviews, fbs, params, &t, 1, &mv);
+        if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+        std::vector<float> ts;
+        for (int it = 0; it < iters + 1; it++) {
+            PCR_HIP(hipEventRecord(e0, ctx->stream));
+            rc = pcr_multiview_launch(mv);
+            if (rc) { pcr_multiview_destroy(mv); return rc; }
+            PCR_HIP(hipEventRecord(e1, ctx->stream));
+            PCR_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            PCR_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (it > 0) ts.push_back(ms);   // the first launch warms up
         }
+        pcr_multiview_destroy(mv);
+        std::sort(ts.begin(), ts.end());
+        const float med = ts[ts.size() / 2];
+        if (med < best_t) { best_t = med; bt = t; }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *best = bt;

@@ -153,8 +153,8 @@ static int launch_group(pcr_server* srv, Slot& sl, const std::vector<uint64_t>& 
     prm.frustum_culling = 1;
     prm.lod_rule = PCR_LOD_SPACING;
     prm.background = PCR_BG_LAS;
-    const pcr_tune t{256, 40, 0};
-    int rc = mv_enqueue_raw(s, c, sl.B, n, max_pix, prm, head.r.mode == PCR_MODE_NODES, t, srv->ctx->cu_count * 8);
+    const pcr_tune t{128, 40, 0, 2048};
+    int rc = mv_enqueue_raw(s, c, sl.B, n, max_pix, prm, head.r.mode == PCR_MODE_NODES, t, srv->ctx->cu_count);
     if (rc) return rc;
     for (int k = 0; k < n; k++) {
         const Req& q = srv->reqs[group[k]];

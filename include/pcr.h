@@ -292,8 +292,10 @@ int pcr_frame_times(pcr_ctx* ctx, int n, float* out5, int* got);
 /* ---- batched multi-view (persistent launch, hipGraph) ------------------- */
 typedef struct pcr_tune {
     int32_t block_size;      /* threads per workgroup: 64,128,256,512            */
-    int32_t points_per_thread;/* 0 = derived from batch size                      */
-    int32_t persistent_groups;/* 0 = derived from the device                      */
+    int32_t points_per_thread;/* untiled: points per thread per work item (x4)    */
+    int32_t persistent_groups;/* untiled: 0 = derived from the device             */
+    int32_t tile_px;         /* 0 = untiled; 1024/2048/4096 = LDS-tiled raster,
+                                one batch per item, block 128 or 256            */
 } pcr_tune;
 
 typedef struct pcr_multiview pcr_multiview;

@@ -53,12 +53,17 @@ def single_view(ctx, cloud, view, prm):
     return out
 
 
+TUNES = {"tiled128": (128, 40, 0, 2048), "tiled256_4096": (256, 40, 0, 4096), "untiled256": (256, 40, 0, 0),
+         "untiled64_160": (64, 160, 0, 0)}
+
+
 @pytest.mark.parametrize("use_graph", [1, 0])
-def test_multiview_equals_single_views(ctx, city, use_graph):
+@pytest.mark.parametrize("tune", sorted(TUNES))
+def test_multiview_equals_single_views(ctx, city, use_graph, tune):
     prm = pcr.params(pcr.MODE_LAS2)
     views = views_for(SIZES * 2, seed=1)
     fbs = [ctx.framebuffer(v.width, v.height) for v in views]
-    mv = pcr.MultiView(ctx, city, views, fbs, prm, use_graph=bool(use_graph))
+    mv = pcr.MultiView(ctx, city, views, fbs, prm, tune=pcr.Tune(*TUNES[tune]), use_graph=bool(use_graph))
     for rnd in range(3):
         if rnd:
             views = views_for(SIZES * 2, seed=1 + rnd)
@@ -96,6 +101,7 @@ def test_autotune_picks_a_valid_config(ctx, city):
     fbs = [ctx.framebuffer(v.width, v.height) for v in views]
     best, ms = pcr.autotune(ctx, city, views, fbs, prm, iters=3)
     assert best.block_size in (64, 128, 256, 512) and best.points_per_thread in (40, 80, 160)
+    assert best.tile_px in (0, 1024, 2048, 4096)
     assert ms > 0
     mv = pcr.MultiView(ctx, city, views, fbs, prm, tune=best)
     mv.launch()

@@ -171,11 +171,11 @@ def test_empty_frame(ctx, terrain):
     assert st.batches_culled == st.batches_total
 
 
-@pytest.mark.parametrize("tile_px", [1024, 2048, 4096])
-def test_tiled_raster_matches_oracle(ctx, terrain, tile_px):
+@pytest.mark.parametrize("tile_px,block", [(1024, 256), (2048, 256), (4096, 256), (2048, 128), (4096, 128)])
+def test_tiled_raster_matches_oracle(ctx, terrain, tile_px, block):
     """LDS-tiled merge gives the same bits as the direct path, far and close views."""
     cloud, lay = terrain
-    prm = pcr.params(pcr.MODE_LAS2, tile_px=tile_px)
+    prm = pcr.params(pcr.MODE_LAS2, tile_px=tile_px, block_size=block)
     views = terrain_views(n=3, seed=21) + [
         pcr.look_at((1000, 1000, 100), (1000, 1010, 60), up=(0, 0, 1), near=0.5, far=4000, width=640, height=360)]
     fb = ctx.framebuffer(640, 360)
