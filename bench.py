@@ -109,6 +109,60 @@ def multiview_leg(ctx, cloud, n_points, nviews, frames, tile_px):
             "bit_exact_vs_plain": bool(exact)}
 
 
+def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
+    """Poisson(rate) render requests against device-resident LOD clouds, served
+    with continuous batching under a framebuffer budget; p50/p99 latency."""
+    import threading
+    clouds, build_s = [], []
+    for k, n in enumerate(sizes):
+        t = time.perf_counter()
+        cfg = pcr.gen_cfg(pcr.SCENE_CITY, n, seed=200 + k, num_buildings=256, terrain_frac=0.2)
+        clouds.append(ctx.octree_generate(cfg, seed=k + 1))
+        build_s.append(round(time.perf_counter() - t, 2))
+    srv = pcr.Server(ctx, budget_bytes=int(budget_gb * (1 << 30)), max_batch=16, num_streams=4)
+    ids = [srv.add_cloud(c) for c in clouds]
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16))
+    arrivals = arrivals[arrivals < seconds]
+    n_req = len(arrivals)
+    which = rng.integers(0, len(clouds), n_req)
+    views = multiview_views(n_req, seed=seed + 1)
+    tickets = [None] * n_req
+    ready = threading.Semaphore(0)
+    start = time.perf_counter()
+
+    def producer():
+        for i in range(n_req):
+            dt = start + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets[i] = srv.submit(ids[which[i]], views[i], pcr.MODE_NODES, lod_px=1.0)
+            ready.release()
+
+    th = threading.Thread(target=producer)
+    th.start()
+    lat = []
+    for i in range(n_req):
+        ready.acquire()
+        _, ms = srv.wait(tickets[i], views[i].width * views[i].height)
+        lat.append(ms)
+    wall = time.perf_counter() - start
+    th.join()
+    st = srv.stats()
+    srv.close()
+    for c in clouds:
+        c.close()
+    lat = np.array(lat)
+    return {"clouds": list(sizes), "octree_build_s": build_s, "requests": n_req, "rate_per_s": rate,
+            "seconds": round(wall, 2), "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+            "latency_ms_max": round(float(lat.max()), 3),
+            "Gpoints_s": round(st.points_rendered / wall / 1e9, 2), "launches": int(st.launches),
+            "mean_requests_per_launch": round(n_req / max(1, st.launches), 2),
+            "fb_budget_GB": budget_gb, "fb_peak_GB": round(st.fb_bytes_peak / (1 << 30), 3),
+            "lod_px": 1.0, "mode": "nodes"}
+
+
 def cpu_baseline(ctx, cloud, views, gpu_fb, prm, frames, n_points):
     """Oracle (multithreaded C) on the host cores for `frames` full frames;
     the GPU framebuffers of the same views are compared bit for bit."""
@@ -146,6 +200,10 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="raster workgroup size (0 = default)")
     ap.add_argument("--mv-views", type=int, default=48, help="views of the batched multi-view leg (0 skips it)")
     ap.add_argument("--mv-frames", type=int, default=8)
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="serving leg duration (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=16.0)
+    ap.add_argument("--serve-clouds", default="10e6,100e6,500e6,1e9")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 device copies of 1 GiB (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -257,6 +315,12 @@ def main():
         print(f"[bench] multiview {out['multiview']}", file=sys.stderr, flush=True)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(ctx, cloud, views, fb, prm, args.cpu_frames, info.num_points)
+    if args.serve_seconds > 0:
+        fb.close()
+        cloud.close()
+        sizes = [int(float(x)) for x in args.serve_clouds.split(",")]
+        out["serving"] = serving_leg(ctx, args.serve_seconds, args.serve_rate, args.serve_budget_gb, sizes)
+        print(f"[bench] serving {out['serving']}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -283,6 +283,14 @@ class Context:
                                                       C.byref(h)))
         return Cloud(self, h)
 
+    def octree_generate(self, cfg: GenCfg, seed=1, root_spacing_div=128.0, max_node_points=20000,
+                        max_depth=12) -> Cloud:
+        """Generate a scene on the device and build its LOD octree there."""
+        h = C.c_void_p()
+        o = OctreeOpts(seed, root_spacing_div, max_node_points, max_depth)
+        check(capi.load().pcr_octree_generate(self.h, C.byref(cfg), C.byref(o), C.byref(h)))
+        return Cloud(self, h)
+
     def framebuffer(self, width, height, kind=FB_KEY64) -> Framebuffer:
         return Framebuffer(self, width, height, kind)
 

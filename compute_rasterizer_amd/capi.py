@@ -134,6 +134,7 @@ SIGNATURES = {
     "pcr_cloud_build_octree_host": (C.c_int, [P, P, P, u64, C.POINTER(OctreeOpts), PP]),
     "pcr_cloud_build_octree_device": (C.c_int, [P, P, P, u64, C.POINTER(OctreeOpts), PP]),
     "pcr_lod_selection": (C.c_int, [P, P, P, u32]),
+    "pcr_octree_generate": (C.c_int, [P, C.POINTER(GenCfg), C.POINTER(OctreeOpts), PP]),
     "pcr_octree_layout_host": (C.c_int, [P, P, u64, C.POINTER(OctreeOpts), P, C.POINTER(u32), P, P, P, P,
                                          C.POINTER(u64)]),
     "pcr_fb_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, PP]),

@@ -124,6 +124,9 @@ int generate_points(pcr_ctx* ctx, const pcr_gen_cfg* cfg, float* xyz, uint32_t* 
 int build_cloud_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, uint64_t n,
                        const pcr_build_opts* opts, pcr_cloud** out);
 int cloud_alloc_batched(pcr_ctx* ctx, uint32_t nb, uint64_t npoints, pcr_cloud* c);
+int device_aabb(pcr_ctx* ctx, const float* xyz, uint64_t n, float mn[3], float mx[3]);
+int build_octree_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, uint64_t n, const pcr_octree_opts* o,
+                        pcr_cloud** out);
 void cloud_free(pcr_cloud* c);
 void encode_batches_host(const float* xyz, uint64_t n, uint32_t bs, pcr_batch* hdr,
                          uint32_t* lo, uint32_t* med, uint32_t* hi);

@@ -275,6 +275,27 @@ int cloud_alloc_batched(pcr_ctx* ctx, uint32_t nb, uint64_t npoints, pcr_cloud* 
     return PCR_OK;
 }
 
+// Float AABB of device points (partial reductions on the device, final on the host).
+int device_aabb(pcr_ctx* ctx, const float* xyz, uint64_t n, float mn[3], float mx[3]) {
+    const int gp = 1024;
+    float* d_part = nullptr;
+    PCR_HIP(hipMalloc(&d_part, sizeof(float) * 6 * gp));
+    k_aabb_partial<<<gp, 256, 0, ctx->stream>>>(xyz, n, d_part);
+    PCR_HIP(hipGetLastError());
+    std::vector<float> part(6 * gp);
+    PCR_HIP(hipMemcpyAsync(part.data(), d_part, sizeof(float) * 6 * gp, hipMemcpyDeviceToHost, ctx->stream));
+    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    PCR_HIP(hipFree(d_part));
+    for (int a = 0; a < 3; a++) { mn[a] = INFINITY; mx[a] = -INFINITY; }
+    for (int b = 0; b < gp; b++)
+        for (int a = 0; a < 3; a++) {
+            const float o = part[6 * b + a], O = part[6 * b + 3 + a];
+            mn[a] = o < mn[a] ? o : mn[a];
+            mx[a] = O > mx[a] ? O : mx[a];
+        }
+    return PCR_OK;
+}
+
 int build_cloud_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, uint64_t n,
                        const pcr_build_opts* opts, pcr_cloud** out) {
     PCR_CHECK(ctx && xyz && rgba && out, "build: null argument");
@@ -285,23 +306,9 @@ int build_cloud_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, uin
     const bool keep16 = opts && opts->keep_points16;
     hipStream_t s = ctx->stream;
 
-    // cloud AABB
-    const int gp = 1024;
-    float* d_part = nullptr;
-    PCR_HIP(hipMalloc(&d_part, sizeof(float) * 6 * gp));
-    k_aabb_partial<<<gp, 256, 0, s>>>(xyz, n, d_part);
-    PCR_HIP(hipGetLastError());
-    std::vector<float> part(6 * gp);
-    PCR_HIP(hipMemcpyAsync(part.data(), d_part, sizeof(float) * 6 * gp, hipMemcpyDeviceToHost, s));
-    PCR_HIP(hipStreamSynchronize(s));
-    PCR_HIP(hipFree(d_part));
-    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < gp; b++)
-        for (int a = 0; a < 3; a++) {
-            const float o = part[6 * b + a], O = part[6 * b + 3 + a];
-            mn[a] = o < mn[a] ? o : mn[a];
-            mx[a] = O > mx[a] ? O : mx[a];
-        }
+    float mn[3], mx[3];
+    const int rc0 = device_aabb(ctx, xyz, n, mn, mx);
+    if (rc0) return rc0;
 
     uint32_t* d_perm = nullptr;
     if (sort) {

@@ -253,13 +253,28 @@ int pcr_cloud_build_octree_host(pcr_ctx* ctx, const float* xyz, const uint32_t* 
 
 int pcr_cloud_build_octree_device(pcr_ctx* ctx, const float* xyz_dev, const uint32_t* rgba_dev, uint64_t n,
                                   const pcr_octree_opts* opts, pcr_cloud** out) {
-    PCR_CHECK(ctx && xyz_dev && rgba_dev && opts && out, "octree_device: null argument");
-    std::vector<float> xyz(3 * n);
-    std::vector<uint32_t> rgba(n);
-    PCR_HIP(hipStreamSynchronize(ctx->stream));
-    PCR_HIP(hipMemcpy(xyz.data(), xyz_dev, 12 * n, hipMemcpyDeviceToHost));
-    PCR_HIP(hipMemcpy(rgba.data(), rgba_dev, 4 * n, hipMemcpyDeviceToHost));
-    return build_octree_host(ctx, xyz.data(), rgba.data(), n, opts, out);
+    return build_octree_device(ctx, xyz_dev, rgba_dev, n, opts, out);
+}
+
+// Generate a synthetic scene on the device and build its octree there.
+int pcr_octree_generate(pcr_ctx* ctx, const pcr_gen_cfg* cfg, const pcr_octree_opts* opts, pcr_cloud** out) {
+    PCR_CHECK(ctx && cfg && opts && out, "octree_generate: null argument");
+    PCR_CHECK(cfg->num_points > 0, "octree_generate: no points");
+    float* xyz = nullptr;
+    uint32_t* rgba = nullptr;
+    PCR_HIP(hipMalloc(&xyz, cfg->num_points * 12));
+    if (hipMalloc(&rgba, cfg->num_points * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(xyz);
+        set_error("octree_generate: device allocation failed");
+        return PCR_E_OOM;
+    }
+    int rc = generate_points(ctx, cfg, xyz, rgba);
+    if (rc == PCR_OK) rc = build_octree_device(ctx, xyz, rgba, cfg->num_points, opts, out);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(xyz);
+    (void)hipFree(rgba);
+    return rc;
 }
 
 int pcr_octree_layout_host(const float* xyz, const uint32_t* rgba, uint64_t n, const pcr_octree_opts* opts,

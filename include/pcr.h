@@ -244,7 +244,10 @@ typedef struct pcr_octree_opts {
  * level/parent/spacing set; node chunks are stored in node order. */
 int pcr_cloud_build_octree_host(pcr_ctx* ctx, const float* xyz_host, const uint32_t* rgba_host,
                                 uint64_t n, const pcr_octree_opts* opts, pcr_cloud** out);
-/* Same, for points already on the device (sorting and level assignment on device). */
+/* Same octree, byte for byte, built on the device from points already there
+ * (Morton sort, per-level subsampling and encoding on the device; only the
+ * per-chunk table visits the host). n < 2^31. */
+int pcr_octree_generate(pcr_ctx* ctx, const pcr_gen_cfg* cfg, const pcr_octree_opts* opts, pcr_cloud** out);
 int pcr_cloud_build_octree_device(pcr_ctx* ctx, const float* xyz_dev, const uint32_t* rgba_dev,
                                   uint64_t n, const pcr_octree_opts* opts, pcr_cloud** out);
 /* Host only (no device work): the same octree as a node array plus planes, for

@@ -186,6 +186,20 @@ def test_tiled_raster_matches_oracle(ctx, terrain, tile_px, block):
         assert np.array_equal(keys, ref), f"{(keys != ref).sum()} keys differ"
 
 
+@pytest.mark.parametrize("n,maxpts", [(400_000, 20000), (150_001, 3000)])
+def test_device_octree_equals_host_octree(ctx, n, maxpts):
+    cfg = pcr.gen_cfg(pcr.SCENE_CITY, n, seed=23, num_buildings=8)
+    xyz, rgba = orc.generate(cfg)
+    host = pcr.octree_layout_host(xyz, rgba, seed=6, max_node_points=maxpts)
+    cloud = ctx.octree_generate(cfg, seed=6, max_node_points=maxpts)
+    dev = cloud.download()
+    assert np.array_equal(dev["batches"].view(np.uint8), host[0].view(np.uint8))
+    for k, h in zip(("lo", "med", "hi", "rgba"), host[1:]):
+        assert np.array_equal(dev[k], h), k
+    assert cloud.info().num_levels == int(host[0]["level"].max()) + 1
+    cloud.close()
+
+
 @pytest.fixture(scope="module")
 def octree(ctx):
     cfg = pcr.gen_cfg(pcr.SCENE_TERRAIN, 400_000, seed=13)
