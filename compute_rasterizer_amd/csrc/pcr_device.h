@@ -145,6 +145,44 @@ __device__ __forceinline__ bool project_xy(const float* __restrict__ t, float x,
     return true;
 }
 
+// The projection constants of one view, with the three addends (t[12],
+// t[13], t[15]) copied into VGPRs once per batch: a gfx950 VALU instruction
+// reads at most one SGPR, so fma(sgpr, v, sgpr) would cost an extra v_mov per
+// point. The arithmetic is the same as project_xy().
+struct Proj {
+    float t0, t4, t8, t1, t5, t9, t3, t7, t11;   // uniform (SGPR)
+    float c12, c13, c15;                        // VGPR copies
+    float fw, fh;
+    int w, h;
+};
+__device__ __forceinline__ float in_vgpr(float x) {
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+__device__ __forceinline__ Proj make_proj(const float* __restrict__ t, int w, int h) {
+    Proj p;
+    p.t0 = t[0]; p.t4 = t[4]; p.t8 = t[8];
+    p.t1 = t[1]; p.t5 = t[5]; p.t9 = t[9];
+    p.t3 = t[3]; p.t7 = t[7]; p.t11 = t[11];
+    p.c12 = in_vgpr(t[12]); p.c13 = in_vgpr(t[13]); p.c15 = in_vgpr(t[15]);
+    p.fw = (float)w; p.fh = (float)h; p.w = w; p.h = h;
+    return p;
+}
+__device__ __forceinline__ bool project_xy(const Proj& p, float x, float y, float z, int& ix, int& iy,
+                                           uint32_t& depth_bits) {
+    const float px = fma_(p.t8, z, fma_(p.t4, y, fma_(p.t0, x, p.c12)));
+    const float py = fma_(p.t9, z, fma_(p.t5, y, fma_(p.t1, x, p.c13)));
+    const float pw = fma_(p.t11, z, fma_(p.t7, y, fma_(p.t3, x, p.c15)));
+    const float nx = px / pw, ny = py / pw;
+    if (!(pw > 0.0f && nx >= -1.0f && nx <= 1.0f && ny >= -1.0f && ny <= 1.0f)) return false;
+    ix = (int)(fma_(nx, 0.5f, 0.5f) * p.fw);
+    iy = (int)(fma_(ny, 0.5f, 0.5f) * p.fh);
+    if (ix >= p.w || iy >= p.h) return false;
+    depth_bits = __float_as_uint(pw);
+    return true;
+}
+
 // Screen rectangle (inclusive, clamped to the image, 1-pixel margin) covered
 // by the projection of an AABB; false when a corner is not in front of the
 // eye or the rectangle is empty. It only routes points to a tile: a point

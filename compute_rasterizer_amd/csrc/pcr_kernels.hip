@@ -232,7 +232,7 @@ static void raster_batched_t(hipStream_t s, const pcr_cloud* c, const pcr_view& 
 // traffic drops to one access per touched pixel per batch.
 template <int LC, bool TILE>
 __device__ __forceinline__ void tile_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi, uint32_t local,
-                                           uint32_t np, uint32_t first, const pcr_view& v,
+                                           uint32_t np, uint32_t first, const Proj& pj,
                                            unsigned long long* __restrict__ fb, unsigned long long* tile, int x0,
                                            int y0, int tw, int th) {
     if (local >= np) return;
@@ -240,11 +240,11 @@ __device__ __forceinline__ void tile_point(const Decode& d, uint32_t lo, uint32_
     decode<LC>(d, lo, med, hi, p);
     int ix, iy;
     uint32_t depth;
-    if (!project_xy(v.transform, p[0], p[1], p[2], v.width, v.height, ix, iy, depth)) return;
+    if (!project_xy(pj, p[0], p[1], p[2], ix, iy, depth)) return;
     const unsigned long long key = ((unsigned long long)depth << 32) | (first + local);
     const uint32_t ux = (uint32_t)(ix - x0), uy = (uint32_t)(iy - y0);
     if (TILE && ux < (uint32_t)tw && uy < (uint32_t)th) lds_min(tile + uy * (uint32_t)tw + ux, key);
-    else fb_min(fb, (uint32_t)ix + (uint32_t)iy * (uint32_t)v.width, key);
+    else fb_min(fb, (uint32_t)ix + (uint32_t)iy * (uint32_t)pj.w, key);
 }
 
 // NT: non-temporal plane loads (a single view streams each plane once);
@@ -258,7 +258,7 @@ __device__ __forceinline__ u32x4 ld_plane(const u32x4* p) {
 template <int BLOCK, int LC, bool TILE, bool NT>
 __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __restrict__ lo,
                                           const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
-                                          uint32_t first, uint32_t np, const pcr_view& v,
+                                          uint32_t first, uint32_t np, const Proj& pj,
                                           unsigned long long* __restrict__ fb, unsigned long long* tile, int x0,
                                           int y0, int tw, int th) {
     const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + first);
@@ -270,24 +270,24 @@ __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __res
         const u32x4 m = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
         const uint32_t base = q * 4u;
-        tile_point<LC, TILE>(d, a.x, m.x, c.x, base + 0, np, first, v, fb, tile, x0, y0, tw, th);
-        tile_point<LC, TILE>(d, a.y, m.y, c.y, base + 1, np, first, v, fb, tile, x0, y0, tw, th);
-        tile_point<LC, TILE>(d, a.z, m.z, c.z, base + 2, np, first, v, fb, tile, x0, y0, tw, th);
-        tile_point<LC, TILE>(d, a.w, m.w, c.w, base + 3, np, first, v, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE>(d, a.x, m.x, c.x, base + 0, np, first, pj, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE>(d, a.y, m.y, c.y, base + 1, np, first, pj, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE>(d, a.z, m.z, c.z, base + 2, np, first, pj, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE>(d, a.w, m.w, c.w, base + 3, np, first, pj, fb, tile, x0, y0, tw, th);
     }
 }
 
 template <int BLOCK, bool TILE, bool NT>
 __device__ __forceinline__ void tile_dispatch(int level, const Decode& d, const uint32_t* __restrict__ lo,
                                               const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
-                                              const pcr_batch& h, const pcr_view& v, unsigned long long* __restrict__ fb,
+                                              const pcr_batch& h, const Proj& pj, unsigned long long* __restrict__ fb,
                                               unsigned long long* tile, int x0, int y0, int tw, int th) {
     if (level == 0)
-        tile_loop<BLOCK, 0, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
+        tile_loop<BLOCK, 0, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th);
     else if (level == 1)
-        tile_loop<BLOCK, 1, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
+        tile_loop<BLOCK, 1, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th);
     else
-        tile_loop<BLOCK, 2, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, v, fb, tile, x0, y0, tw, th);
+        tile_loop<BLOCK, 2, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th);
 }
 
 // One batch of the tiled raster; `bi` and `level` are workgroup-uniform.
@@ -298,6 +298,7 @@ __device__ __forceinline__ void tiled_batch(uint32_t bi, int level, const pcr_ba
                                             unsigned long long* __restrict__ fb, unsigned long long* tile) {
     const pcr_batch h = b[bi];
     const Decode d = make_decode(level, h.min, h.max);
+    const Proj pj = make_proj(v.transform, v.width, v.height);
     int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
     bool use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1);
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
@@ -308,17 +309,20 @@ __device__ __forceinline__ void tiled_batch(uint32_t bi, int level, const pcr_ba
         const int n = tw * th;
         for (int i = threadIdx.x; i < n; i += BLOCK) tile[i] = PCR_FB_CLEAR;
         __syncthreads();
-        tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, v, fb, tile, x0, y0, tw, th);
+        tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, pj, fb, tile, x0, y0, tw, th);
         __syncthreads();
+        // row = i / tw as a multiply-high by ceil(2^32 / tw): exact because
+        // i < 2^13 and tw <= 2^13 keep the error below 1/tw
+        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
         for (int i = threadIdx.x; i < n; i += BLOCK) {
             const unsigned long long k = tile[i];
             if (k != PCR_FB_CLEAR) {
-                const int uy = i / tw, ux = i - uy * tw;
-                fb_min(fb, (uint32_t)(x0 + ux) + (uint32_t)(y0 + uy) * (uint32_t)v.width, k);
+                const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
+                fb_min(fb, (uint32_t)x0 + ux + ((uint32_t)y0 + uy) * (uint32_t)v.width, k);
             }
         }
     } else {
-        tile_dispatch<BLOCK, false, NT>(level, d, lo, med, hi, h, v, fb, nullptr, 0, 0, 0, 0);
+        tile_dispatch<BLOCK, false, NT>(level, d, lo, med, hi, h, pj, fb, nullptr, 0, 0, 0, 0);
     }
 }
 

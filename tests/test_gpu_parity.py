@@ -76,12 +76,12 @@ def test_plain16_config1(ctx):
     assert (ref != pcr.FB_CLEAR).sum() > 100_000
 
 
-@pytest.mark.parametrize("mode", [pcr.MODE_LAS, pcr.MODE_LAS2])
+@pytest.mark.parametrize("mode,tile", [(pcr.MODE_LAS, 0), (pcr.MODE_LAS2, 0), (pcr.MODE_LAS2, 2048)])
 @pytest.mark.parametrize("culling", [1, 0])
-def test_batched_matches_oracle(ctx, terrain, mode, culling):
+def test_batched_matches_oracle(ctx, terrain, mode, tile, culling):
     cloud, lay = terrain
     fb = ctx.framebuffer(640, 360)
-    prm = pcr.params(mode, culling=culling)
+    prm = pcr.params(mode, culling=culling, tile_px=tile)
     for view in terrain_views():
         ctx.render(cloud, view, prm, fb)
         ctx.resolve(cloud, prm, fb)
