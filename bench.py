@@ -109,6 +109,35 @@ def multiview_leg(ctx, cloud, n_points, nviews, frames, tile_px):
             "bit_exact_vs_plain": bool(exact)}
 
 
+def layout_sweep(ctx, n_points, batch_sizes, nviews, frames):
+    """Autotuned multi-view frame time per batch size (each a separate layout of
+    the same 1e9-point scene; the batch AABBs, hence the decoded points, differ)."""
+    views = multiview_views(nviews)
+    fbs = [ctx.framebuffer(v.width, v.height) for v in views]
+    prm = pcr.params(pcr.MODE_LAS2)
+    res = []
+    for bs in batch_sizes:
+        cloud = ctx.generate(pcr.gen_cfg(pcr.SCENE_CITY, n_points, seed=42), batch_size=bs)
+        best, best_ms = pcr.autotune(ctx, cloud, views, fbs, prm, iters=3)
+        mv = pcr.MultiView(ctx, cloud, views, fbs, prm, tune=best)
+        mv.launch()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            mv.launch()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) / frames * 1e3
+        res.append({"batch_size": bs, "batched_ms_per_frame": round(ms, 3),
+                    "batched_Gpoints_s": round(n_points * nviews / ms / 1e6, 1),
+                    "tune": {"block_size": best.block_size, "points_per_thread": best.points_per_thread,
+                             "tile_px": best.tile_px}})
+        mv.close()
+        cloud.close()
+    for fb in fbs:
+        fb.close()
+    return res
+
+
 def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
     """Poisson(rate) render requests against device-resident LOD clouds, served
     with continuous batching under a framebuffer budget; p50/p99 latency."""
@@ -131,29 +160,80 @@ def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
     ready = threading.Semaphore(0)
     start = time.perf_counter()
 
+    errors = []
+
     def producer():
-        for i in range(n_req):
-            dt = start + arrivals[i] - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            tickets[i] = srv.submit(ids[which[i]], views[i], pcr.MODE_NODES, lod_px=1.0)
+        try:
+            for i in range(n_req):
+                dt = start + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[i] = srv.submit(ids[which[i]], views[i], pcr.MODE_NODES, lod_px=1.0)
+                ready.release()
+        except Exception as e:  # surface the error in the main thread instead of blocking it
+            errors.append(e)
             ready.release()
 
     th = threading.Thread(target=producer)
     th.start()
     lat = []
     for i in range(n_req):
-        ready.acquire()
+        if not ready.acquire(timeout=60):
+            raise RuntimeError(f"serving: request {i} was not submitted within 60 s")
+        if errors:
+            raise errors[0]
         _, ms = srv.wait(tickets[i], views[i].width * views[i].height)
         lat.append(ms)
     wall = time.perf_counter() - start
     th.join()
     st = srv.stats()
+
+    # Saturated phase: the same kind of requests all submitted at once, served
+    # with continuous batching, against rendering them one at a time through
+    # the plain per-view path; a sample of images is compared bit for bit.
+    nb = min(n_req, 512)
+    bviews = multiview_views(nb, seed=seed + 2)
+    bwhich = rng.integers(0, len(clouds), nb)
+    prm = pcr.params(pcr.MODE_NODES, lod_px=1.0)
+    fbs = {}
+    t0 = time.perf_counter()
+    for i in range(nb):
+        v = bviews[i]
+        fb = fbs.get((v.width, v.height)) or fbs.setdefault((v.width, v.height), ctx.framebuffer(v.width, v.height))
+        ctx.render(clouds[bwhich[i]], v, prm, fb)
+        ctx.resolve(clouds[bwhich[i]], prm, fb)
+    ctx.synchronize()
+    plain_s = time.perf_counter() - t0
+    launches0 = srv.stats().launches
+    t0 = time.perf_counter()
+    bt = [srv.submit(ids[bwhich[i]], bviews[i], pcr.MODE_NODES, lod_px=1.0) for i in range(nb)]
+    imgs = {}
+    for i, t in enumerate(bt):
+        img, _ = srv.wait(t, bviews[i].width * bviews[i].height)
+        if i % 64 == 0:
+            imgs[i] = img
+    served_s = time.perf_counter() - t0
+    burst_launches = srv.stats().launches - launches0
+    exact = True
+    for i, img in imgs.items():
+        v = bviews[i]
+        fb = fbs[(v.width, v.height)]
+        ctx.render(clouds[bwhich[i]], v, prm, fb)
+        ctx.resolve(clouds[bwhich[i]], prm, fb)
+        exact = exact and bool(np.array_equal(fb.rgba(), img))
+    for fb in fbs.values():
+        fb.close()
+    pts = float(sum(int(clouds[k].info().num_points) for k in bwhich))
+    burst = {"requests": nb, "plain_s": round(plain_s, 3), "served_s": round(served_s, 3),
+             "plain_Gpoints_s": round(pts / plain_s / 1e9, 1), "served_Gpoints_s": round(pts / served_s / 1e9, 1),
+             "speedup": round(plain_s / served_s, 3), "launches": int(burst_launches),
+             "mean_requests_per_launch": round(nb / max(1, burst_launches), 2),
+             "bit_exact_vs_plain_sample": exact}
     srv.close()
     for c in clouds:
         c.close()
     lat = np.array(lat)
-    return {"clouds": list(sizes), "octree_build_s": build_s, "requests": n_req, "rate_per_s": rate,
+    return {"burst": burst, "clouds": list(sizes), "octree_build_s": build_s, "requests": n_req, "rate_per_s": rate,
             "seconds": round(wall, 2), "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
             "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
             "latency_ms_max": round(float(lat.max()), 3),
@@ -200,6 +280,8 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="raster workgroup size (0 = default)")
     ap.add_argument("--mv-views", type=int, default=48, help="views of the batched multi-view leg (0 skips it)")
     ap.add_argument("--mv-frames", type=int, default=8)
+    ap.add_argument("--mv-batch-sizes", default="5120,20480",
+                    help="extra layouts for the multi-view autotune sweep ('' skips it)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="serving leg duration (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
@@ -315,6 +397,12 @@ def main():
         print(f"[bench] multiview {out['multiview']}", file=sys.stderr, flush=True)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(ctx, cloud, views, fb, prm, args.cpu_frames, info.num_points)
+    if args.mv_views > 0 and args.mv_batch_sizes:
+        fb.close()
+        cloud.close()
+        sizes = [int(x) for x in args.mv_batch_sizes.split(",")]
+        out["multiview"]["layout_sweep"] = layout_sweep(ctx, args.points, sizes, args.mv_views, args.mv_frames)
+        print(f"[bench] layout sweep {out['multiview']['layout_sweep']}", file=sys.stderr, flush=True)
     if args.serve_seconds > 0:
         fb.close()
         cloud.close()

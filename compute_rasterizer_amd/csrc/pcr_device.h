@@ -43,6 +43,34 @@ __device__ inline bool frustum_visible(const float* __restrict__ m, const float 
     return true;
 }
 
+// The same test split in two: plane i of the view (normal and constant, four
+// floats), computed once per workgroup, then the p-vertex test per AABB. The
+// operations are those of frustum_visible(), so the results are identical.
+__device__ inline void frustum_plane(const float* __restrict__ m, int i, float out[4]) {
+    const int axis = i >> 1;
+    const bool minus = (i == 0 || i == 3 || i == 4);
+    float p[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float a = m[4 * k + 3], b = m[4 * k + axis];
+        p[k] = minus ? a - b : a + b;
+    }
+    const float len = __builtin_sqrtf(fma_(p[2], p[2], fma_(p[1], p[1], p[0] * p[0])));
+    out[0] = p[0] / len; out[1] = p[1] / len; out[2] = p[2] / len; out[3] = p[3] / len;
+}
+__device__ inline bool planes_visible(const float (*pl)[4], const float mn[3], const float mx[3]) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const float nx = pl[i][0], ny = pl[i][1], nz = pl[i][2], c = pl[i][3];
+        const float vx = nx > 0.0f ? mx[0] : mn[0];
+        const float vy = ny > 0.0f ? mx[1] : mn[1];
+        const float vz = nz > 0.0f ? mx[2] : mn[2];
+        const float d = fma_(nz, vz, fma_(ny, vy, nx * vx)) + c;
+        if (d < 0.0f) return false;
+    }
+    return true;
+}
+
 // Screen-space length (pixels) of a view-space x offset `len` placed at the
 // AABB centre (compute_loop_las/render.cs:199-215).
 __device__ inline float projected_px(const float mn[3], const float mx[3], float len,

@@ -84,12 +84,15 @@ __global__ __launch_bounds__(256) void k_select(const pcr_batch* __restrict__ b,
                                                 int culling, int rule, float lod_px, float first,
                                                 uint8_t* __restrict__ levels, uint32_t* __restrict__ list,
                                                 FrameState* __restrict__ st) {
+    __shared__ float planes[6][4];   // the view's frustum planes, computed once per workgroup
+    if (threadIdx.x < 6) frustum_plane(v.transform_frustum, (int)threadIdx.x, planes[threadIdx.x]);
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     int level = kCulled;
     uint32_t np = 0;
     if (i < nb) {
         const pcr_batch h = b[i];
-        bool vis = !culling || frustum_visible(v.transform_frustum, h.min, h.max);
+        bool vis = !culling || planes_visible(planes, h.min, h.max);
         if (LOD && vis) {
             if (rule == PCR_LOD_SIZE80) {
                 vis = precision_level(h.min, h.max, v, 80.0f) < 4;
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(256) void k_select(const pcr_batch* __restrict__ b,
                 int32_t p = h.parent;
                 while (vis && p >= 0) {
                     const pcr_batch a = b[p];
-                    vis = (!culling || frustum_visible(v.transform_frustum, a.min, a.max)) &&
+                    vis = (!culling || planes_visible(planes, a.min, a.max)) &&
                           projected_px(a.min, a.max, a.spacing, v.view_world, v.proj, v.width, v.height) > lod_px;
                     p = a.parent;
                 }

@@ -45,6 +45,10 @@ __global__ __launch_bounds__(256) void k_mv_select(const pcr_batch* __restrict__
                                                    int lod, int rule, const float* __restrict__ lod_px,
                                                    uint8_t* __restrict__ levels, uint8_t* __restrict__ best,
                                                    uint32_t* __restrict__ list, MVState* __restrict__ st) {
+    __shared__ float planes[kMaxViews][6][4];   // every view's frustum planes, once per workgroup
+    for (int t = (int)threadIdx.x; t < 6 * nviews; t += (int)blockDim.x)
+        frustum_plane(views[t / 6].transform_frustum, t % 6, planes[t / 6][t % 6]);
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     int top = kCulled;
     if (i < nb) {
@@ -52,7 +56,7 @@ __global__ __launch_bounds__(256) void k_mv_select(const pcr_batch* __restrict__
         const float first = lod ? 80.0f : 100.0f;
         for (int v = 0; v < nviews; v++) {
             const pcr_view& vw = views[v];
-            bool vis = !culling || frustum_visible(vw.transform_frustum, h.min, h.max);
+            bool vis = !culling || planes_visible(planes[v], h.min, h.max);
             if (lod && vis) {
                 if (rule == PCR_LOD_SIZE80) {
                     vis = precision_level(h.min, h.max, vw, 80.0f) < 4;
@@ -60,7 +64,7 @@ __global__ __launch_bounds__(256) void k_mv_select(const pcr_batch* __restrict__
                     int32_t p = h.parent;
                     while (vis && p >= 0) {
                         const pcr_batch a = b[p];
-                        vis = (!culling || frustum_visible(vw.transform_frustum, a.min, a.max)) &&
+                        vis = (!culling || planes_visible(planes[v], a.min, a.max)) &&
                               projected_px(a.min, a.max, a.spacing, vw.view_world, vw.proj, vw.width, vw.height) >
                                   lod_px[v];
                         p = a.parent;

@@ -66,26 +66,54 @@ __global__ void k_oct_flags(const uint64_t* __restrict__ K, uint64_t n, int l, i
     }
 }
 
-__global__ void k_oct_count(const uint32_t* __restrict__ nid, const int8_t* __restrict__ lev, uint64_t n,
-                            uint32_t* __restrict__ ncount) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        if (lev[i] < 0) atomicAdd(ncount + nid[i] - 1, 1u);
+// one atomic per wave for a per-thread count (keeps a single counter uncontended)
+__device__ __forceinline__ void wave_add(unsigned long long* p, uint32_t v) {
+    uint32_t s = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (__lane_id() == 0 && s) atomicAdd(p, (unsigned long long)s);
 }
 
-__global__ void k_oct_leaf(const uint32_t* __restrict__ nid, int8_t* __restrict__ lev, uint64_t n,
-                           const uint32_t* __restrict__ ncount, uint32_t maxpts, int l, int last,
-                           unsigned long long* __restrict__ assigned) {
+__global__ void k_oct_unassigned(const int8_t* __restrict__ lev, uint64_t n, uint32_t* __restrict__ uf) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        uf[i] = lev[i] < 0 ? 1u : 0u;
+}
+
+__global__ void k_oct_nstart(const uint32_t* __restrict__ nflag, const uint32_t* __restrict__ nid, uint64_t n,
+                             uint32_t* __restrict__ ns) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if (nflag[i]) ns[nid[i] - 1] = (uint32_t)i;
+}
+
+// A node whose unassigned points fit in maxpts (or any node of the last
+// level) keeps them all. The count is a difference of the inclusive prefix
+// sum U of "unassigned" flags over the node's range [ns[node], next start).
+__global__ void k_oct_leaf(const uint32_t* __restrict__ nid, const uint32_t* __restrict__ U,
+                           const uint32_t* __restrict__ ns, int8_t* __restrict__ lev, uint64_t n, uint32_t maxpts,
+                           int l, int last, unsigned long long* __restrict__ assigned) {
+    const uint32_t nnodes = nid[n - 1];
     uint32_t mine = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        if (lev[i] < 0 && (last || ncount[nid[i] - 1] <= maxpts)) { lev[i] = (int8_t)l; mine++; }
-    if (mine) atomicAdd(assigned, (unsigned long long)mine);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (lev[i] >= 0) continue;
+        const uint32_t node = nid[i] - 1, s = ns[node];
+        const uint64_t e = node + 1 < nnodes ? ns[node + 1] : n;
+        const uint32_t cnt = U[e - 1] - (s ? U[s - 1] : 0u);
+        if (last || cnt <= maxpts) { lev[i] = (int8_t)l; mine++; }
+    }
+    wave_add(assigned, mine);
 }
 
+// lowest priority per subsampling cell; the plain read skips most atomics
+// (it can only be stale-high: the value only decreases)
 __global__ void k_oct_cellmin(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ cid,
                               const int8_t* __restrict__ lev, uint64_t n, uint64_t pkey,
                               unsigned long long* __restrict__ cellmin) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        if (lev[i] < 0) atomicMin(cellmin + cid[i] - 1, (unsigned long long)rnd(pkey, perm[i]));
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (lev[i] >= 0) continue;
+        const unsigned long long p = rnd(pkey, perm[i]);
+        unsigned long long* c = cellmin + cid[i] - 1;
+        if (p < *c) atomicMin(c, p);
+    }
 }
 
 __global__ void k_oct_cellwin(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ cid,
@@ -100,7 +128,7 @@ __global__ void k_oct_assign(const uint32_t* __restrict__ cid, int8_t* __restric
     uint32_t mine = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         if (lev[i] < 0 && cellwin[cid[i] - 1] == (uint32_t)i) { lev[i] = (int8_t)l; mine++; }
-    if (mine) atomicAdd(assigned, (unsigned long long)mine);
+    wave_add(assigned, mine);
 }
 
 __global__ void k_oct_iota(uint32_t* __restrict__ v, uint64_t n) {
@@ -298,9 +326,12 @@ int build_octree_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, ui
         k_oct_flags<<<G, 256, 0, s>>>(K, n, l, gbits, nflag, cflag);
         PCR_HIP(hipcub::DeviceScan::InclusiveSum(stemp, tscan, nflag, nid, (int)n, s));
         PCR_HIP(hipcub::DeviceScan::InclusiveSum(stemp, tscan, cflag, cid, (int)n, s));
-        k_oct_fill32<<<G, 256, 0, s>>>(ncount, n, 0u);
-        k_oct_count<<<G, 256, 0, s>>>(nid, lev, n, ncount);
-        k_oct_leaf<<<G, 256, 0, s>>>(nid, lev, n, ncount, maxpts, l, l == max_depth, assigned);
+        // per-node unassigned counts as prefix-sum differences: cflag is free
+        // once cid exists, cellwin holds the node starts until the leaf pass
+        k_oct_unassigned<<<G, 256, 0, s>>>(lev, n, cflag);
+        PCR_HIP(hipcub::DeviceScan::InclusiveSum(stemp, tscan, cflag, ncount, (int)n, s));
+        k_oct_nstart<<<G, 256, 0, s>>>(nflag, nid, n, cellwin);
+        k_oct_leaf<<<G, 256, 0, s>>>(nid, ncount, cellwin, lev, n, maxpts, l, l == max_depth, assigned);
         k_oct_fill64<<<G, 256, 0, s>>>(cellmin, n, ~0ull);
         k_oct_fill32<<<G, 256, 0, s>>>(cellwin, n, ~0u);
         k_oct_cellmin<<<G, 256, 0, s>>>(perm, cid, lev, n, pkey, cellmin);
