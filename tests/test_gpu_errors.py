@@ -1,0 +1,94 @@
+"""GPU: invalid inputs fail with a status code and a message (the reference
+skipped silently), and repeated frames give identical bits (atomic min is
+order independent, so any schedule must produce the same framebuffer)."""
+import numpy as np
+import pytest
+
+import compute_rasterizer_amd as pcr
+from compute_rasterizer_amd import capi
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(ctx):
+    xyz, rgba = orc.generate(pcr.gen_cfg(pcr.SCENE_TERRAIN, 20_000, seed=8))
+    b, lo, med, hi = pcr.encode_batches_host(xyz, 1024)
+    return xyz, rgba, b, lo, med, hi
+
+
+def expect_invalid(fn, text):
+    with pytest.raises(pcr.PcrError) as e:
+        fn()
+    assert e.value.code == capi.PCR_E_INVALID
+    assert text in str(e.value)
+
+
+def test_upload_validation(ctx, small):
+    xyz, rgba, b, lo, med, hi = small
+    bad = b.copy()
+    bad[3]["first_point"] += 2
+    expect_invalid(lambda: ctx.upload_batched(bad, lo, med, hi, rgba), "multiple of 4")
+    bad = b.copy()
+    bad[-1]["num_points"] += 64
+    expect_invalid(lambda: ctx.upload_batched(bad, lo, med, hi, rgba), "exceeds")
+    bad = b.copy()
+    bad[2]["parent"] = 5
+    expect_invalid(lambda: ctx.upload_batched(bad, lo, med, hi, rgba), "parent")
+
+
+def test_render_validation(ctx, small):
+    xyz, rgba, b, lo, med, hi = small
+    cloud = ctx.upload_batched(b, lo, med, hi, rgba)
+    pts = ctx.upload_points16(xyz, rgba)
+    v = pcr.look_at((1000, -200, 400), (1000, 1000, 50), up=(0, 0, 1), near=1, far=5000, width=320, height=200)
+    fb = ctx.framebuffer(320, 200)
+    hfb = ctx.framebuffer(320, 200, pcr.FB_HQS)
+    small_fb = ctx.framebuffer(100, 100)
+    expect_invalid(lambda: ctx.render(cloud, v, pcr.params(pcr.MODE_LAS2), small_fb), "differs from framebuffer")
+    expect_invalid(lambda: ctx.render(cloud, v, pcr.params(pcr.MODE_HQS), fb), "HQS framebuffer")
+    expect_invalid(lambda: ctx.render(cloud, v, pcr.params(pcr.MODE_LAS2), hfb), "KEY64 framebuffer")
+    expect_invalid(lambda: ctx.render(pts, v, pcr.params(pcr.MODE_LAS2), fb), "no batched layout")
+    expect_invalid(lambda: ctx.render(cloud, v, pcr.params(pcr.MODE_PLAIN16), fb), "16 B/pt")
+    expect_invalid(lambda: ctx.render(cloud, v, pcr.params(pcr.MODE_LAS2, tile_px=3000), fb), "tile_px")
+    expect_invalid(lambda: ctx.render(cloud, v, pcr.params(pcr.MODE_LAS2, block_size=96), fb), "block_size")
+    expect_invalid(lambda: ctx.render(cloud, v, pcr.params(7), fb), "unknown mode")
+    # after the errors the context still renders correctly
+    prm = pcr.params(pcr.MODE_LAS2)
+    ctx.render(cloud, v, prm, fb)
+    ref, _ = orc.render_batched(b, lo, med, hi, v, 1)
+    assert np.array_equal(fb.keys(), ref)
+
+
+def test_multiview_validation(ctx, small):
+    xyz, rgba, b, lo, med, hi = small
+    cloud = ctx.upload_batched(b, lo, med, hi, rgba)
+    v = pcr.look_at((1000, -200, 400), (1000, 1000, 50), up=(0, 0, 1), near=1, far=5000, width=64, height=64)
+    fbs = [ctx.framebuffer(64, 64) for _ in range(65)]
+    expect_invalid(lambda: pcr.MultiView(ctx, cloud, [v] * 65, fbs, pcr.params(pcr.MODE_LAS2)), "views")
+    expect_invalid(lambda: pcr.MultiView(ctx, cloud, [v], fbs[:1], pcr.params(pcr.MODE_HQS)), "mode")
+    expect_invalid(lambda: pcr.MultiView(ctx, cloud, [v], fbs[:1], pcr.params(pcr.MODE_LAS2),
+                                         tune=pcr.Tune(512, 40, 0, 2048)), "tile_px")
+    mv = pcr.MultiView(ctx, cloud, [v, v], fbs[:2], pcr.params(pcr.MODE_LAS2))
+    v2 = pcr.look_at((0, 0, 10), (1, 1, 0), width=32, height=32)
+    expect_invalid(lambda: mv.set_views([v2, v2]), "changed size")
+    mv.close()
+
+
+@pytest.mark.parametrize("tile", [0, 2048])
+def test_repeated_frames_identical(ctx, tile):
+    cloud = ctx.generate(pcr.gen_cfg(pcr.SCENE_CITY, 3_000_000, seed=31, num_buildings=32))
+    v = pcr.look_at((1000, 200, 300), (1000, 1000, 20), up=(0, 0, 1), near=1, far=5000, width=1280, height=720)
+    fb = ctx.framebuffer(1280, 720)
+    prm = pcr.params(pcr.MODE_LAS2, tile_px=tile)
+    first = None
+    for _ in range(5):
+        ctx.render(cloud, v, prm, fb)
+        ctx.resolve(cloud, prm, fb)
+        k = fb.keys()
+        if first is None:
+            first = k
+        assert np.array_equal(k, first)
+    assert (first != pcr.FB_CLEAR).sum() > 10_000
+    cloud.close()
