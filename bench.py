@@ -190,7 +190,8 @@ def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
 
     # Saturated phase: the same kind of requests all submitted at once, served
     # with continuous batching, against rendering them one at a time through
-    # the plain per-view path; a sample of images is compared bit for bit.
+    # the plain per-view path. Both hand every image to the host; a sample of
+    # images is compared bit for bit.
     nb = min(n_req, 512)
     bviews = multiview_views(nb, seed=seed + 2)
     bwhich = rng.integers(0, len(clouds), nb)
@@ -202,6 +203,7 @@ def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
         fb = fbs.get((v.width, v.height)) or fbs.setdefault((v.width, v.height), ctx.framebuffer(v.width, v.height))
         ctx.render(clouds[bwhich[i]], v, prm, fb)
         ctx.resolve(clouds[bwhich[i]], prm, fb)
+        fb.rgba()
     ctx.synchronize()
     plain_s = time.perf_counter() - t0
     launches0 = srv.stats().launches

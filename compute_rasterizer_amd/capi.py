@@ -149,6 +149,7 @@ SIGNATURES = {
     "pcr_resolve": (C.c_int, [P, P, C.POINTER(RenderParams), P]),
     "pcr_stats_get": (C.c_int, [P, C.POINTER(Stats)]),
     "pcr_frame_times": (C.c_int, [P, C.c_int, P, C.POINTER(C.c_int)]),
+    "pcr_selftest_divide": (C.c_int, [P, P, P, P, C.c_uint64, P]),
     "pcr_multiview_create": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(C.c_void_p),
                                        C.POINTER(RenderParams), C.POINTER(Tune), C.c_int, PP]),
     "pcr_multiview_set_views": (C.c_int, [P, C.POINTER(View), C.c_int]),

@@ -28,13 +28,13 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 }
 
 static bool valid_view(const pcr_view* v) {
-    return v && v->width > 0 && v->height > 0 && (uint64_t)v->width * (uint64_t)v->height < (1ull << 32);
+    return v && v->width > 0 && v->height > 0 && v->width <= PCR_MAX_DIM && v->height <= PCR_MAX_DIM;
 }
 
-static int clear_fb(hipStream_t s, pcr_fb* fb) {
+static int clear_fb(hipStream_t s, pcr_fb* fb, FrameState* st = nullptr) {
     const uint64_t npix = (uint64_t)fb->width * fb->height;
-    if (fb->kind == PCR_FB_KEY64) return launch_fill64(s, fb->d_keys, npix, PCR_FB_CLEAR);
-    int rc = launch_fill32(s, fb->d_depth, npix, PCR_DEPTH_CLEAR);
+    if (fb->kind == PCR_FB_KEY64) return launch_fill64(s, fb->d_keys, npix, PCR_FB_CLEAR, st);
+    int rc = launch_fill32(s, fb->d_depth, npix, PCR_DEPTH_CLEAR, st);
     if (rc) return rc;
     return launch_fill32(s, fb->d_acc, 4 * npix, 0u);
 }
@@ -110,7 +110,8 @@ int pcr_view_look_at(const double eye[3], const double target[3], const double u
                      double near_plane, double far_plane, const double origin[3], int width, int height,
                      pcr_view* out) {
     PCR_CHECK(eye && target && up && origin && out, "view_look_at: null argument");
-    PCR_CHECK(width > 0 && height > 0, "view_look_at: bad size %dx%d", width, height);
+    PCR_CHECK(width > 0 && height > 0 && width <= PCR_MAX_DIM && height <= PCR_MAX_DIM,
+              "view_look_at: bad size %dx%d", width, height);
     PCR_CHECK(near_plane > 0 && far_plane > near_plane, "view_look_at: bad clip planes");
     double f[3] = {target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]};
     const double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
@@ -163,7 +164,8 @@ int pcr_view_look_at(const double eye[3], const double target[3], const double u
 int pcr_view_from_matrices(const double view[16], const double proj[16], const double origin[3], int width,
                            int height, pcr_view* out) {
     PCR_CHECK(view && proj && origin && out, "view_from_matrices: null argument");
-    PCR_CHECK(width > 0 && height > 0, "view_from_matrices: bad size %dx%d", width, height);
+    PCR_CHECK(width > 0 && height > 0 && width <= PCR_MAX_DIM && height <= PCR_MAX_DIM,
+              "view_from_matrices: bad size %dx%d", width, height);
     double W[16] = {0}, VW[16], PVW[16];
     W[0] = W[5] = W[10] = W[15] = 1.0;
     W[12] = origin[0]; W[13] = origin[1]; W[14] = origin[2];
@@ -343,8 +345,9 @@ int pcr_cloud_destroy(pcr_cloud* c) {
 // ------------------------------------------------------------ framebuffers
 int pcr_fb_create(pcr_ctx* ctx, int width, int height, int kind, pcr_fb** out) {
     PCR_CHECK(ctx && out, "fb_create: null argument");
-    PCR_CHECK(width > 0 && height > 0 && (uint64_t)width * height < (1ull << 31), "fb_create: bad size %dx%d",
-              width, height);
+    PCR_CHECK(width > 0 && height > 0 && width <= PCR_MAX_DIM && height <= PCR_MAX_DIM &&
+                  (uint64_t)width * height < (1ull << 31),
+              "fb_create: bad size %dx%d", width, height);
     PCR_CHECK(kind == PCR_FB_KEY64 || kind == PCR_FB_HQS, "fb_create: bad kind %d", kind);
     const uint64_t npix = (uint64_t)width * height;
     pcr_fb* f = new pcr_fb();
@@ -440,9 +443,12 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
     hipEvent_t* ev = ctx->ring[ctx->frames_timed % PCR_TIMING_RING];
     if (ctx->timed) { ctx->frames_timed++; PCR_HIP(hipEventRecord(ev[0], s)); }
     int rc = PCR_OK;
-    if (params->clear_before && (rc = clear_fb(s, fb))) return rc;
+    if (params->clear_before) {
+        if ((rc = clear_fb(s, fb, ctx->d_state))) return rc;
+    } else if ((rc = launch_frame_reset(s, ctx->d_state))) {
+        return rc;
+    }
     if (ctx->timed) PCR_HIP(hipEventRecord(ev[1], s));
-    if ((rc = launch_frame_reset(s, ctx->d_state))) return rc;
     if (!pts16) {
         rc = launch_select(s, cloud, *view, *params, ctx->d_state, mode == PCR_MODE_NODES ? 80.0f : 100.0f,
                            mode == PCR_MODE_NODES);
@@ -522,6 +528,28 @@ int pcr_frame_times(pcr_ctx* ctx, int n, float* out5, int* got) {
         }
     }
     *got = (int)avail;
+    return PCR_OK;
+}
+
+int pcr_selftest_divide(pcr_ctx* ctx, const float* num_x, const float* num_y, const float* den, uint64_t n,
+                        float* out2) {
+    PCR_CHECK(ctx && num_x && num_y && den && out2, "selftest_divide: null argument");
+    PCR_CHECK(n <= (1ull << 28), "selftest_divide: n %llu exceeds 2^28", (unsigned long long)n);
+    if (n == 0) return PCR_OK;
+    float* d = nullptr;
+    PCR_HIP(hipMalloc(&d, n * 20));
+    int rc = PCR_OK;
+    hipError_t e = hipMemcpy(d, num_x, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, num_y, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 2 * n, den, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = launch_selftest_divide(ctx->stream, d, d + n, d + 2 * n, n, d + 3 * n);
+        if (rc == PCR_OK) e = hipMemcpyAsync(out2, d + 3 * n, n * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (rc == PCR_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
+    (void)hipFree(d);
+    if (rc != PCR_OK) return rc;
+    PCR_HIP(e);
     return PCR_OK;
 }
 

@@ -138,6 +138,35 @@ __device__ __forceinline__ void decode(const Decode& d, uint32_t lo, uint32_t me
     }
 }
 
+// px/pw and py/pw for pw > 0, each the correctly rounded IEEE quotient.
+//
+// gfx950 divides in fp32 as: scale the operands (v_div_scale), y = rcp(d),
+// y = fma(fma(-d, y, 1), y, y), q = n*y, q = fma(fma(-d, q, n), y, q), then
+// q = fma(fma(-d, q, n), y, q) (v_div_fmas), then patch special values
+// (v_div_fixup). When d and |n| lie in [2^-40, 2^40] the scaling and the
+// patch are identities: operands, reciprocal and quotient are normal, the
+// exponent gap is below 96 and the numerator exponent above 23. There the
+// code below is the same operations on the same values, with the reciprocal
+// step shared by the two numerators. Every other operand (a zero or tiny
+// numerator, an extreme depth) takes the compiler's division.
+__device__ __forceinline__ void div_by_w(float px, float py, float pw, float& nx, float& ny) {
+    const float lo = fminf(fminf(fabsf(px), fabsf(py)), pw);
+    const float hi = fmaxf(fmaxf(fabsf(px), fabsf(py)), pw);
+    if (lo >= 0x1p-40f && hi <= 0x1p40f) {
+        const float y0 = __builtin_amdgcn_rcpf(pw);
+        const float y = fma_(fma_(-pw, y0, 1.0f), y0, y0);
+        float q = px * y;
+        q = fma_(fma_(-pw, q, px), y, q);
+        nx = fma_(fma_(-pw, q, px), y, q);
+        q = py * y;
+        q = fma_(fma_(-pw, q, py), y, q);
+        ny = fma_(fma_(-pw, q, py), y, q);
+    } else {
+        nx = px / pw;
+        ny = py / pw;
+    }
+}
+
 // Project a point with `t` (column-major), returning the pixel and the depth
 // bits (clip w = view-space depth) when it lands inside the image
 // (compute_loop_las/render.cs:400-409; truncated pixel == width/height and
@@ -147,8 +176,10 @@ __device__ __forceinline__ bool project(const float* __restrict__ t, float x, fl
     const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
     const float py = fma_(t[9], z, fma_(t[5], y, fma_(t[1], x, t[13])));
     const float pw = fma_(t[11], z, fma_(t[7], y, fma_(t[3], x, t[15])));
-    const float nx = px / pw, ny = py / pw;
-    if (!(pw > 0.0f && nx >= -1.0f && nx <= 1.0f && ny >= -1.0f && ny <= 1.0f)) return false;
+    if (!(pw > 0.0f)) return false;
+    float nx, ny;
+    div_by_w(px, py, pw, nx, ny);
+    if (!(nx >= -1.0f && nx <= 1.0f && ny >= -1.0f && ny <= 1.0f)) return false;
     const float sx = fma_(nx, 0.5f, 0.5f) * (float)w;
     const float sy = fma_(ny, 0.5f, 0.5f) * (float)h;
     const int ix = (int)sx, iy = (int)sy;
@@ -158,25 +189,10 @@ __device__ __forceinline__ bool project(const float* __restrict__ t, float x, fl
     return true;
 }
 
-// Same as project() but returning the pixel column and row.
-__device__ __forceinline__ bool project_xy(const float* __restrict__ t, float x, float y, float z, int w, int h,
-                                           int& ix, int& iy, uint32_t& depth_bits) {
-    const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
-    const float py = fma_(t[9], z, fma_(t[5], y, fma_(t[1], x, t[13])));
-    const float pw = fma_(t[11], z, fma_(t[7], y, fma_(t[3], x, t[15])));
-    const float nx = px / pw, ny = py / pw;
-    if (!(pw > 0.0f && nx >= -1.0f && nx <= 1.0f && ny >= -1.0f && ny <= 1.0f)) return false;
-    ix = (int)(fma_(nx, 0.5f, 0.5f) * (float)w);
-    iy = (int)(fma_(ny, 0.5f, 0.5f) * (float)h);
-    if (ix >= w || iy >= h) return false;
-    depth_bits = __float_as_uint(pw);
-    return true;
-}
-
 // The projection constants of one view, with the three addends (t[12],
 // t[13], t[15]) copied into VGPRs once per batch: a gfx950 VALU instruction
 // reads at most one SGPR, so fma(sgpr, v, sgpr) would cost an extra v_mov per
-// point. The arithmetic is the same as project_xy().
+// point. The arithmetic is the same as project().
 struct Proj {
     float t0, t4, t8, t1, t5, t9, t3, t7, t11;   // uniform (SGPR)
     float c12, c13, c15;                        // VGPR copies
@@ -202,8 +218,10 @@ __device__ __forceinline__ bool project_xy(const Proj& p, float x, float y, floa
     const float px = fma_(p.t8, z, fma_(p.t4, y, fma_(p.t0, x, p.c12)));
     const float py = fma_(p.t9, z, fma_(p.t5, y, fma_(p.t1, x, p.c13)));
     const float pw = fma_(p.t11, z, fma_(p.t7, y, fma_(p.t3, x, p.c15)));
-    const float nx = px / pw, ny = py / pw;
-    if (!(pw > 0.0f && nx >= -1.0f && nx <= 1.0f && ny >= -1.0f && ny <= 1.0f)) return false;
+    if (!(pw > 0.0f)) return false;
+    float nx, ny;
+    div_by_w(px, py, pw, nx, ny);
+    if (!(nx >= -1.0f && nx <= 1.0f && ny >= -1.0f && ny <= 1.0f)) return false;
     ix = (int)(fma_(nx, 0.5f, 0.5f) * p.fw);
     iy = (int)(fma_(ny, 0.5f, 0.5f) * p.fh);
     if (ix >= p.w || iy >= p.h) return false;

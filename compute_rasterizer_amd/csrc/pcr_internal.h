@@ -81,8 +81,9 @@ struct pcr_fb {
 namespace pcr {
 
 // kernels (pcr_kernels.hip)
-int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v);
-int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
+// st: also reset that frame state (the clear that opens a frame)
+int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v, FrameState* st = nullptr);
+int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v, FrameState* st = nullptr);
 int launch_frame_reset(hipStream_t s, FrameState* st);
 int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pcr_render_params& p,
                   FrameState* st, float first_threshold, bool lod);
@@ -94,6 +95,8 @@ int launch_raster_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v,
 int launch_hqs_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb);
 int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pcr_fb* fb);
 int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, pcr_fb* fb);
+int launch_selftest_divide(hipStream_t s, const float* nx, const float* ny, const float* den, uint64_t n,
+                           float* out2);
 
 // multi-view frame (pcr_multiview.hip), shared with the server
 struct MVState {

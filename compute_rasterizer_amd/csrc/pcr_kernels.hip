@@ -36,20 +36,25 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // ------------------------------------------------------------------ fills
-__global__ void k_fill64(unsigned long long* __restrict__ p, uint64_t n, unsigned long long v) {
+// The frame's counters reset with the clear (one launch fewer per frame):
+// thread 0 of workgroup 0 zeroes them when `st` is given.
+__device__ __forceinline__ void frame_reset(FrameState* st) {
+    st->list_count = 0; st->queue_head = 0; st->culled = 0;
+    for (int i = 0; i < 5; i++) st->level_count[i] = 0;
+    st->points_drawn = 0; st->plane_bytes = 0;
+}
+__global__ void k_fill64(unsigned long long* __restrict__ p, uint64_t n, unsigned long long v, FrameState* st) {
+    if (st && blockIdx.x == 0 && threadIdx.x == 0) frame_reset(st);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
-__global__ void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+__global__ void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v, FrameState* st) {
+    if (st && blockIdx.x == 0 && threadIdx.x == 0) frame_reset(st);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 __global__ void k_frame_reset(FrameState* st) {
-    if (threadIdx.x == 0) {
-        st->list_count = 0; st->queue_head = 0; st->culled = 0;
-        for (int i = 0; i < 5; i++) st->level_count[i] = 0;
-        st->points_drawn = 0; st->plane_bytes = 0;
-    }
+    if (threadIdx.x == 0) frame_reset(st);
 }
 
 static int grid_for(uint64_t n, int block) {
@@ -59,13 +64,13 @@ static int grid_for(uint64_t n, int block) {
     return (int)g;
 }
 
-int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v) {
-    k_fill64<<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v, FrameState* st) {
+    k_fill64<<<grid_for(n, 256), 256, 0, s>>>(p, n, v, st);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
 }
-int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v) {
-    k_fill32<<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v, FrameState* st) {
+    k_fill32<<<grid_for(n, 256), 256, 0, s>>>(p, n, v, st);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
 }
@@ -77,17 +82,24 @@ int launch_frame_reset(hipStream_t s, FrameState* st) {
 
 // ----------------------------------------------------------------- select
 // One thread per batch/node: frustum cull, LOD cut, precision level. Writes
-// levels[i] (255 = not drawn), appends drawn batches to `list` with one
-// atomic per wave, and accumulates per-frame statistics.
+// levels[i] (255 = not drawn), appends drawn batches to `list` and
+// accumulates per-frame statistics. Counts are combined across the
+// workgroup's 16 waves in LDS first, so the frame's shared counters see one
+// list atomic and eight statistics atomics per 1024 batches.
+constexpr int kSelectBlock = 1024;
 template <bool LOD>
-__global__ __launch_bounds__(256) void k_select(const pcr_batch* __restrict__ b, uint32_t nb, pcr_view v,
-                                                int culling, int rule, float lod_px, float first,
-                                                uint8_t* __restrict__ levels, uint32_t* __restrict__ list,
-                                                FrameState* __restrict__ st) {
+__global__ __launch_bounds__(kSelectBlock) void k_select(const pcr_batch* __restrict__ b, uint32_t nb, pcr_view v,
+                                                         int culling, int rule, float lod_px, float first,
+                                                         uint8_t* __restrict__ levels, uint32_t* __restrict__ list,
+                                                         FrameState* __restrict__ st) {
+    constexpr int kWaves = kSelectBlock / 64;
     __shared__ float planes[6][4];   // the view's frustum planes, computed once per workgroup
+    __shared__ uint32_t s_keep[kWaves];
+    __shared__ unsigned long long s_stat[8][kWaves];   // culled, level 0..4, points, bytes
+    __shared__ uint32_t s_base;
     if (threadIdx.x < 6) frustum_plane(v.transform_frustum, (int)threadIdx.x, planes[threadIdx.x]);
     __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * kSelectBlock + threadIdx.x;
     int level = kCulled;
     uint32_t np = 0;
     if (i < nb) {
@@ -111,38 +123,54 @@ __global__ __launch_bounds__(256) void k_select(const pcr_batch* __restrict__ b,
     }
     const bool keep = level != kCulled;
     const unsigned long long m = __ballot(keep);
-    const uint32_t lane = __lane_id();
-    if (m) {
-        const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&st->list_count, (uint32_t)__popcll(m));
-        base = __shfl(base, (int)leader);
-        if (keep) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
-    }
-    const unsigned long long culled = __ballot(i < nb && !keep);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t culled = (uint32_t)__popcll(__ballot(i < nb && !keep));
     uint32_t cnt[5];
 #pragma unroll
     for (int L = 0; L < 5; L++) cnt[L] = (uint32_t)__popcll(__ballot(level == L));
     const unsigned long long pts = wave_sum_u64(np);
     const unsigned long long bytes = wave_sum_u64(keep ? (unsigned long long)np * 4ull * planes_for_level(level) : 0ull);
     if (lane == 0) {
-        if (culled) atomicAdd(&st->culled, (uint32_t)__popcll(culled));
+        s_keep[wave] = (uint32_t)__popcll(m);
+        s_stat[0][wave] = culled;
 #pragma unroll
-        for (int L = 0; L < 5; L++) if (cnt[L]) atomicAdd(&st->level_count[L], cnt[L]);
-        if (pts) atomicAdd(&st->points_drawn, pts);
-        if (bytes) atomicAdd(&st->plane_bytes, bytes);
+        for (int L = 0; L < 5; L++) s_stat[1 + L][wave] = cnt[L];
+        s_stat[6][wave] = pts;
+        s_stat[7][wave] = bytes;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < kWaves; w++) total += s_keep[w];
+        s_base = total ? atomicAdd(&st->list_count, total) : 0u;
+    } else if (threadIdx.x >= 64 && threadIdx.x < 72) {
+        const int k = (int)threadIdx.x - 64;
+        unsigned long long sum = 0;
+        for (int w = 0; w < kWaves; w++) sum += s_stat[k][w];
+        if (sum) {
+            if (k == 0) atomicAdd(&st->culled, (uint32_t)sum);
+            else if (k <= 5) atomicAdd(&st->level_count[k - 1], (uint32_t)sum);
+            else if (k == 6) atomicAdd(&st->points_drawn, sum);
+            else atomicAdd(&st->plane_bytes, sum);
+        }
+    }
+    __syncthreads();
+    if (keep) {
+        uint32_t base = s_base;
+        for (uint32_t w = 0; w < wave; w++) base += s_keep[w];
+        list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
     }
 }
 
 int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pcr_render_params& p,
                   FrameState* st, float first, bool lod) {
-    const int grid = (int)((c->num_batches + 255) / 256);
+    const int grid = (int)((c->num_batches + kSelectBlock - 1) / kSelectBlock);
     if (grid == 0) return PCR_OK;
     if (lod)
-        k_select<true><<<grid, 256, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling, p.lod_rule,
+        k_select<true><<<grid, kSelectBlock, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling, p.lod_rule,
                                             p.lod_px, first, c->d_levels, c->d_list, st);
     else
-        k_select<false><<<grid, 256, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling, p.lod_rule,
+        k_select<false><<<grid, kSelectBlock, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling, p.lod_rule,
                                              p.lod_px, first, c->d_levels, c->d_list, st);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
@@ -224,6 +252,28 @@ template <int BLOCK, int VEC>
 static void raster_batched_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb) {
     k_raster_batched<BLOCK, VEC><<<c->num_batches, BLOCK, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
                                                                   c->d_hi, v, fb);
+}
+
+// --------------------------------------------------------- division check
+__global__ __launch_bounds__(256) void k_selftest_divide(const float* __restrict__ nx, const float* __restrict__ ny,
+                                                         const float* __restrict__ den, uint64_t n,
+                                                         float* __restrict__ out2) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const float d = den[i];
+        float qx = 0.0f, qy = 0.0f;
+        if (d > 0.0f) div_by_w(nx[i], ny[i], d, qx, qy);
+        out2[2 * i] = qx;
+        out2[2 * i + 1] = qy;
+    }
+}
+
+int launch_selftest_divide(hipStream_t s, const float* nx, const float* ny, const float* den, uint64_t n,
+                           float* out2) {
+    if (n == 0) return PCR_OK;
+    const uint64_t g = (n + 255) / 256;
+    k_selftest_divide<<<(int)(g < 4096 ? g : 4096), 256, 0, s>>>(nx, ny, den, n, out2);
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
 }
 
 // ------------------------------------------------------------ tiled raster
@@ -350,7 +400,13 @@ __global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restr
         __syncthreads();
         const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
         if (item >= count) break;
-        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[item]);
+        // Consecutive queue items map to list slots spread over the whole list
+        // (multiplying by a prime above any count, modulo the count, permutes
+        // the slots), so the batches in flight at one time come from distant
+        // parts of the Morton order and cover different screen regions: fewer
+        // of their atomics meet on the same framebuffer lines.
+        const uint32_t slot = (uint32_t)(((uint64_t)item * 2654435761ull) % count);
+        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[slot]);
         const int level = __builtin_amdgcn_readfirstlane(levels[bi]);
         tiled_batch<BLOCK, TILE_PIX, true>(bi, level, b, lo, med, hi, v, fb, tile);
         __syncthreads();   // s_item and the tile are reused by the next item

@@ -66,7 +66,7 @@ struct Arena {
     }
 };
 
-enum ReqState { kPending = 0, kInFlight = 1, kDone = 2, kCancelled = 3 };
+enum ReqState { kPending = 0, kInFlight = 1, kDone = 2, kCancelled = 3, kTaking = 4 };
 
 struct Req {
     pcr_request r{};
@@ -307,16 +307,27 @@ int pcr_server_submit(pcr_server* srv, const pcr_request* req, uint64_t* ticket)
     return PCR_OK;
 }
 
-static int poll_locked(pcr_server* srv, uint64_t ticket, uint32_t* rgba, double* latency_ms) {
+// The image is copied out with the server lock released (the worker keeps
+// launching meanwhile); kTaking keeps the request and its arena range alive
+// until the copy is done.
+static int take_locked(pcr_server* srv, std::unique_lock<std::mutex>& lk, uint64_t ticket, uint32_t* rgba,
+                       double* latency_ms) {
     auto it = srv->reqs.find(ticket);
     PCR_CHECK(it != srv->reqs.end(), "server_poll: unknown ticket %llu", (unsigned long long)ticket);
     Req& r = it->second;
     if (r.state == kCancelled) return PCR_E_CANCELLED;
+    PCR_CHECK(r.state != kTaking, "server_poll: ticket %llu is being taken by another call",
+              (unsigned long long)ticket);
     if (r.state != kDone) return PCR_E_NOT_READY;
-    if (rgba) memcpy(rgba, srv->h_arena + r.off / 3, 4 * r.npix);
     if (latency_ms) *latency_ms = r.t_done - r.t_submit;
-    srv->arena.release(r.off, r.bytes);
-    srv->reqs.erase(it);
+    r.state = kTaking;
+    const unsigned char* src = srv->h_arena + r.off / 3;
+    const uint64_t npix = r.npix, off = r.off, bytes = r.bytes;
+    lk.unlock();
+    if (rgba) memcpy(rgba, src, 4 * npix);
+    lk.lock();
+    srv->arena.release(off, bytes);
+    srv->reqs.erase(ticket);
     srv->work_cv.notify_one();
     return PCR_OK;
 }
@@ -324,14 +335,14 @@ static int poll_locked(pcr_server* srv, uint64_t ticket, uint32_t* rgba, double*
 int pcr_server_poll(pcr_server* srv, uint64_t ticket, uint32_t* rgba, double* latency_ms) {
     PCR_CHECK(srv, "server_poll: null server");
     std::unique_lock<std::mutex> lk(srv->mu);
-    return poll_locked(srv, ticket, rgba, latency_ms);
+    return take_locked(srv, lk, ticket, rgba, latency_ms);
 }
 
 int pcr_server_wait(pcr_server* srv, uint64_t ticket, uint32_t* rgba, double* latency_ms) {
     PCR_CHECK(srv, "server_wait: null server");
     std::unique_lock<std::mutex> lk(srv->mu);
     for (;;) {
-        const int rc = poll_locked(srv, ticket, rgba, latency_ms);
+        const int rc = take_locked(srv, lk, ticket, rgba, latency_ms);
         if (rc != PCR_E_NOT_READY) return rc;
         if (srv->worker_error) { set_error("server_wait: the server stopped after a HIP error"); return PCR_E_HIP; }
         srv->done_cv.wait_for(lk, std::chrono::milliseconds(2));
@@ -353,7 +364,7 @@ int pcr_server_cancel(pcr_server* srv, uint64_t ticket) {
     } else if (r.state == kDone) {
         srv->arena.release(r.off, r.bytes);
         srv->reqs.erase(it);
-    }
+    }   // kTaking: another call is copying the finished image out; nothing to cancel
     return PCR_OK;
 }
 

@@ -46,6 +46,7 @@ extern "C" {
 #define PCR_DEPTH_CLEAR     0xFFFFFFFFu
 #define PCR_BG_LAS          0x00443322u    /* compute_loop_las/resolve.cs:141 */
 #define PCR_BG_BLACK        0xFF000000u
+#define PCR_MAX_DIM         32768          /* framebuffer and view width/height limit */
 
 /*
  * 64-byte batch / node header.
@@ -291,6 +292,12 @@ int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out);   /* synchronises */
  * GL timestamps for the same purpose). Synchronises; *got = frames written. */
 #define PCR_TIMING_RING 512
 int pcr_frame_times(pcr_ctx* ctx, int n, float* out5, int* got);
+/* Device check of the projection's division (no counterpart in the
+ * reference; for tests): for i < n with den[i] > 0, out2[2i] and out2[2i+1]
+ * receive num_x[i]/den[i] and num_y[i]/den[i] as the rasterizer computes
+ * them, which must equal the IEEE quotients. Host arrays; synchronises. */
+int pcr_selftest_divide(pcr_ctx* ctx, const float* num_x, const float* num_y, const float* den, uint64_t n,
+                        float* out2);
 
 /* ---- batched multi-view (persistent launch, hipGraph) ------------------- */
 typedef struct pcr_tune {
