@@ -211,9 +211,11 @@ def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
     bt = [srv.submit(ids[bwhich[i]], bviews[i], pcr.MODE_NODES, lod_px=1.0) for i in range(nb)]
     imgs = {}
     for i, t in enumerate(bt):
-        img, _ = srv.wait(t, bviews[i].width * bviews[i].height)
+        # zero copy: the image is read in the server's pinned result memory
+        img, _ = srv.wait_pinned(t, bviews[i].width * bviews[i].height)
         if i % 64 == 0:
-            imgs[i] = img
+            imgs[i] = img.copy()
+        srv.release(t)
     served_s = time.perf_counter() - t0
     burst_launches = srv.stats().launches - launches0
     exact = True
@@ -305,7 +307,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py: this build renders on one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not "
-              f"supported. Multi-GPU sharding is described in DESIGN.md.", file=sys.stderr)
+              f"supported (sharding batches across GPUs with a min reduce-scatter of the framebuffer is "
+              f"designed, not built).", file=sys.stderr)
         sys.exit(2)
 
     ctx = pcr.Context(0)

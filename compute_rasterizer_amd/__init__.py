@@ -32,14 +32,14 @@ def look_at(eye, target, up=(0.0, 1.0, 0.0), fovy=60.0, near=0.1, far=1000.0, or
 
 
 def params(mode=MODE_LAS2, culling=1, background=None, lod_rule=LOD_SPACING, lod_px=1.0, block_size=0,
-           clear_before=1, time_stages=0, tile_px=None) -> RenderParams:
+           clear_before=1, time_stages=0, tile_px=None, batch_begin=0, batch_end=0) -> RenderParams:
     """Render parameters; LAS2 and NODES default to the LDS-tiled raster (2048-px tile)."""
     if background is None:
         background = BG_BLACK if mode in (MODE_PLAIN16,) else BG_LAS
     if tile_px is None:
         tile_px = 2048 if mode in (MODE_LAS2, MODE_NODES) else 0
     return RenderParams(mode, culling, lod_rule, lod_px, background, block_size, clear_before, time_stages,
-                        tile_px)
+                        tile_px, batch_begin, batch_end)
 
 
 def gen_cfg(scene, n, seed=1, extent=2000.0, amplitude=150.0, jitter=0.5, num_buildings=256,
@@ -222,6 +222,18 @@ class Server:
         lat = C.c_double()
         check(capi.load().pcr_server_wait(self.h, ticket, ptr(out), C.byref(lat)))
         return out, lat.value
+
+    def wait_pinned(self, ticket: int, npix: int):
+        """(image, latency_ms) without a copy: the image is a view of the
+        server's pinned result memory, valid until release(ticket)."""
+        p = C.c_void_p()
+        lat = C.c_double()
+        check(capi.load().pcr_server_wait_pinned(self.h, ticket, C.byref(p), C.byref(lat)))
+        img = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), shape=(npix,))
+        return img, lat.value
+
+    def release(self, ticket: int):
+        check(capi.load().pcr_server_release(self.h, ticket))
 
     def cancel(self, ticket: int):
         check(capi.load().pcr_server_cancel(self.h, ticket))

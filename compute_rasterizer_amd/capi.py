@@ -60,7 +60,18 @@ class RenderParams(C.Structure):
     _fields_ = [("mode", C.c_int32), ("frustum_culling", C.c_int32), ("lod_rule", C.c_int32),
                 ("lod_px", C.c_float), ("background", C.c_uint32), ("block_size", C.c_int32),
                 ("clear_before", C.c_int32), ("time_stages", C.c_int32), ("tile_px", C.c_int32),
-                ("pad", C.c_int32 * 3)]
+                ("batch_begin", C.c_uint32), ("batch_end", C.c_uint32), ("pad", C.c_int32)]
+
+
+class LasInfo(C.Structure):
+    _fields_ = [("num_points", C.c_uint64), ("offset_to_points", C.c_uint32), ("record_length", C.c_uint16),
+                ("point_format", C.c_uint8), ("version_major", C.c_uint8), ("version_minor", C.c_uint8),
+                ("rgb_offset", C.c_uint8), ("pad", C.c_uint8 * 6), ("scale", C.c_double * 3),
+                ("offset", C.c_double * 3), ("min", C.c_double * 3), ("max", C.c_double * 3)]
+
+
+class LasFile(C.Structure):
+    _fields_ = [("bytes", C.c_void_p), ("size", C.c_uint64)]
 
 
 class Stats(C.Structure):
@@ -148,6 +159,8 @@ SIGNATURES = {
     "pcr_render": (C.c_int, [P, P, C.POINTER(View), C.POINTER(RenderParams), P]),
     "pcr_resolve": (C.c_int, [P, P, C.POINTER(RenderParams), P]),
     "pcr_stats_get": (C.c_int, [P, C.POINTER(Stats)]),
+    "pcr_las_parse_header": (C.c_int, [P, C.c_uint64, P]),
+    "pcr_cloud_from_las": (C.c_int, [P, P, C.c_int, P, C.POINTER(C.c_void_p)]),
     "pcr_frame_times": (C.c_int, [P, C.c_int, P, C.POINTER(C.c_int)]),
     "pcr_selftest_divide": (C.c_int, [P, P, P, P, C.c_uint64, P]),
     "pcr_multiview_create": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(C.c_void_p),
@@ -162,6 +175,8 @@ SIGNATURES = {
     "pcr_server_submit": (C.c_int, [P, C.POINTER(Request), C.POINTER(u64)]),
     "pcr_server_poll": (C.c_int, [P, u64, P, dp]),
     "pcr_server_wait": (C.c_int, [P, u64, P, dp]),
+    "pcr_server_wait_pinned": (C.c_int, [P, C.c_uint64, C.POINTER(C.c_void_p), C.POINTER(C.c_double)]),
+    "pcr_server_release": (C.c_int, [P, C.c_uint64]),
     "pcr_server_cancel": (C.c_int, [P, u64]),
     "pcr_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
     "pcr_server_destroy": (C.c_int, [P]),

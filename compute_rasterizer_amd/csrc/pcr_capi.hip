@@ -242,8 +242,6 @@ int pcr_cloud_upload_batched(pcr_ctx* ctx, const pcr_batch* batches, uint32_t nb
     int max_level = -1;
     for (uint32_t b = 0; b < nb; b++) {
         const pcr_batch& h = batches[b];
-        PCR_CHECK(h.first_point % 4 == 0, "upload_batched: batch %u first_point %u is not a multiple of 4", b,
-                  h.first_point);
         PCR_CHECK((uint64_t)h.first_point + h.num_points <= npoints,
                   "upload_batched: batch %u range [%u, +%u) exceeds %llu points", b, h.first_point, h.num_points,
                   (unsigned long long)npoints);
@@ -435,6 +433,11 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
               "render: bad lod_rule %d", params->lod_rule);
     PCR_CHECK(params->tile_px == 0 || params->tile_px == 1024 || params->tile_px == 2048 || params->tile_px == 4096,
               "render: tile_px %d not in {0,1024,2048,4096}", params->tile_px);
+    PCR_CHECK(params->batch_end == 0 ? params->batch_begin == 0
+                                      : params->batch_begin <= params->batch_end &&
+                                            params->batch_end <= (pts16 ? 0u : cloud->num_batches),
+              "render: batch range [%u, %u) outside the cloud's %u batches", params->batch_begin, params->batch_end,
+              pts16 ? 0u : cloud->num_batches);
     const int block = params->block_size;
     PCR_CHECK(block == 0 || block == 64 || block == 128 || block == 256 || block == 512,
               "render: block_size %d not in {64,128,256,512}", block);

@@ -90,6 +90,7 @@ constexpr int kSelectBlock = 1024;
 template <bool LOD>
 __global__ __launch_bounds__(kSelectBlock) void k_select(const pcr_batch* __restrict__ b, uint32_t nb, pcr_view v,
                                                          int culling, int rule, float lod_px, float first,
+                                                         uint32_t b0, uint32_t b1,
                                                          uint8_t* __restrict__ levels, uint32_t* __restrict__ list,
                                                          FrameState* __restrict__ st) {
     constexpr int kWaves = kSelectBlock / 64;
@@ -100,9 +101,10 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(const pcr_batch* __rest
     if (threadIdx.x < 6) frustum_plane(v.transform_frustum, (int)threadIdx.x, planes[threadIdx.x]);
     __syncthreads();
     const uint32_t i = blockIdx.x * kSelectBlock + threadIdx.x;
+    const bool in_range = i < nb && i >= b0 && i < b1;   // [b0, b1): the batches of this draw
     int level = kCulled;
     uint32_t np = 0;
-    if (i < nb) {
+    if (in_range) {
         const pcr_batch h = b[i];
         bool vis = !culling || planes_visible(planes, h.min, h.max);
         if (LOD && vis) {
@@ -119,12 +121,12 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(const pcr_batch* __rest
             }
         }
         if (vis) { level = precision_level(h.min, h.max, v, first); np = h.num_points; }
-        levels[i] = (uint8_t)level;
     }
+    if (i < nb) levels[i] = (uint8_t)level;
     const bool keep = level != kCulled;
     const unsigned long long m = __ballot(keep);
     const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-    const uint32_t culled = (uint32_t)__popcll(__ballot(i < nb && !keep));
+    const uint32_t culled = (uint32_t)__popcll(__ballot(in_range && !keep));
     uint32_t cnt[5];
 #pragma unroll
     for (int L = 0; L < 5; L++) cnt[L] = (uint32_t)__popcll(__ballot(level == L));
@@ -166,12 +168,14 @@ int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pc
                   FrameState* st, float first, bool lod) {
     const int grid = (int)((c->num_batches + kSelectBlock - 1) / kSelectBlock);
     if (grid == 0) return PCR_OK;
+    const uint32_t b0 = p.batch_begin, b1 = p.batch_end ? p.batch_end : c->num_batches;
     if (lod)
         k_select<true><<<grid, kSelectBlock, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling, p.lod_rule,
-                                            p.lod_px, first, c->d_levels, c->d_list, st);
+                                                     p.lod_px, first, b0, b1, c->d_levels, c->d_list, st);
     else
-        k_select<false><<<grid, kSelectBlock, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling, p.lod_rule,
-                                             p.lod_px, first, c->d_levels, c->d_list, st);
+        k_select<false><<<grid, kSelectBlock, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling,
+                                                      p.lod_rule, p.lod_px, first, b0, b1, c->d_levels, c->d_list,
+                                                      st);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
 }
@@ -196,15 +200,16 @@ __device__ __forceinline__ void raster_vec4(const Decode& d, const uint32_t* __r
                                             const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                             uint32_t first, uint32_t np, const pcr_view& v,
                                             unsigned long long* __restrict__ fb) {
-    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + first);
-    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + first);
-    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + first);
-    const uint32_t nq = (np + 3u) >> 2;
+    const uint32_t lead = first & 3u;   // see tile_loop()
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
+    const uint32_t nq = (np + lead + 3u) >> 2;
     for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
         const u32x4 a = ld_stream(lo4 + q);
         const u32x4 m = LC <= 1 ? ld_stream(med4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 c = LC == 0 ? ld_stream(hi4 + q) : u32x4{0, 0, 0, 0};
-        const uint32_t base = q * 4u;
+        const uint32_t base = q * 4u - lead;
         raster_point<LC>(d, a.x, m.x, c.x, base + 0, np, first, v, fb);
         raster_point<LC>(d, a.y, m.y, c.y, base + 1, np, first, v, fb);
         raster_point<LC>(d, a.z, m.z, c.z, base + 2, np, first, v, fb);
@@ -314,15 +319,21 @@ __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __res
                                           uint32_t first, uint32_t np, const Proj& pj,
                                           unsigned long long* __restrict__ fb, unsigned long long* tile, int x0,
                                           int y0, int tw, int th) {
-    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + first);
-    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + first);
-    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + first);
-    const uint32_t nq = (np + 3u) >> 2;
+    // A batch may start at any word: quads are read from the aligned word
+    // at or below `first`, and the `lead` words in front of the batch get a
+    // local index that wraps past np, so the local < np test drops them as
+    // it drops the padding behind the last point. first + local is the
+    // point's index either way.
+    const uint32_t lead = first & 3u;
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
+    const uint32_t nq = (np + lead + 3u) >> 2;
     for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
         const u32x4 a = ld_plane<NT>(lo4 + q);
         const u32x4 m = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
-        const uint32_t base = q * 4u;
+        const uint32_t base = q * 4u - lead;
         tile_point<LC, TILE>(d, a.x, m.x, c.x, base + 0, np, first, pj, fb, tile, x0, y0, tw, th);
         tile_point<LC, TILE>(d, a.y, m.y, c.y, base + 1, np, first, pj, fb, tile, x0, y0, tw, th);
         tile_point<LC, TILE>(d, a.z, m.z, c.z, base + 2, np, first, pj, fb, tile, x0, y0, tw, th);
@@ -606,17 +617,18 @@ __device__ __forceinline__ void hqs_vec4(const Decode& d, const uint32_t* __rest
                                          const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                          const uint32_t* __restrict__ rgba, uint32_t first, uint32_t np,
                                          const pcr_view& v, uint32_t* __restrict__ depth, uint32_t* __restrict__ acc) {
-    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + first);
-    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + first);
-    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + first);
-    const u32x4* c4 = reinterpret_cast<const u32x4*>(rgba + first);
-    const uint32_t nq = (np + 3u) >> 2;
+    const uint32_t lead = first & 3u;   // see tile_loop()
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
+    const u32x4* c4 = reinterpret_cast<const u32x4*>(rgba + (first - lead));
+    const uint32_t nq = (np + lead + 3u) >> 2;
     for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
         const u32x4 a = ld_stream(lo4 + q);
         const u32x4 m = LC <= 1 ? ld_stream(med4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 c = LC == 0 ? ld_stream(hi4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 col = PASS == 1 ? ld_stream(c4 + q) : u32x4{0, 0, 0, 0};
-        const uint32_t base = q * 4u;
+        const uint32_t base = q * 4u - lead;
         hqs_point<PASS, LC>(d, a.x, m.x, c.x, col.x, base + 0, np, v, depth, acc);
         hqs_point<PASS, LC>(d, a.y, m.y, c.y, col.y, base + 1, np, v, depth, acc);
         hqs_point<PASS, LC>(d, a.z, m.z, c.z, col.z, base + 2, np, v, depth, acc);

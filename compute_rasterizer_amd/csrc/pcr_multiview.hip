@@ -154,10 +154,12 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster(const pcr_batch* __restrict
             const uint32_t cnt = min(chunk_pts, h.num_points - start);
             const uint32_t first = h.first_point + start;
             const int top = best[bi];
-            const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + first);
-            const u32x4* med4 = reinterpret_cast<const u32x4*>(med + first);
-            const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + first);
-            const uint32_t nq = (cnt + 3u) >> 2;
+            // unaligned batch starts as in the single-view tiled raster
+            const uint32_t lead = first & 3u;
+            const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
+            const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
+            const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
+            const uint32_t nq = (cnt + lead + 3u) >> 2;
             for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
                 const u32x4 a = __builtin_nontemporal_load(lo4 + q);
                 const u32x4 m = top <= 1 ? __builtin_nontemporal_load(med4 + q) : u32x4{0, 0, 0, 0};
@@ -168,9 +170,9 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster(const pcr_batch* __restrict
                     const pcr_view& v = views[vi];
                     const Decode d = make_decode(lv, h.min, h.max);
                     unsigned long long* fb = fbs[vi];
-                    if (lv == 0) mv_points4<0>(d, a, m, c, q * 4u, cnt, first, v, fb);
-                    else if (lv == 1) mv_points4<1>(d, a, m, c, q * 4u, cnt, first, v, fb);
-                    else mv_points4<2>(d, a, m, c, q * 4u, cnt, first, v, fb);
+                    if (lv == 0) mv_points4<0>(d, a, m, c, q * 4u - lead, cnt, first, v, fb);
+                    else if (lv == 1) mv_points4<1>(d, a, m, c, q * 4u - lead, cnt, first, v, fb);
+                    else mv_points4<2>(d, a, m, c, q * 4u - lead, cnt, first, v, fb);
                 }
             }
         }

@@ -66,7 +66,7 @@ struct Arena {
     }
 };
 
-enum ReqState { kPending = 0, kInFlight = 1, kDone = 2, kCancelled = 3, kTaking = 4 };
+enum ReqState { kPending = 0, kInFlight = 1, kDone = 2, kCancelled = 3, kTaking = 4, kHeld = 5 };
 
 struct Req {
     pcr_request r{};
@@ -316,7 +316,7 @@ static int take_locked(pcr_server* srv, std::unique_lock<std::mutex>& lk, uint64
     PCR_CHECK(it != srv->reqs.end(), "server_poll: unknown ticket %llu", (unsigned long long)ticket);
     Req& r = it->second;
     if (r.state == kCancelled) return PCR_E_CANCELLED;
-    PCR_CHECK(r.state != kTaking, "server_poll: ticket %llu is being taken by another call",
+    PCR_CHECK(r.state != kTaking && r.state != kHeld, "server_poll: ticket %llu is already taken",
               (unsigned long long)ticket);
     if (r.state != kDone) return PCR_E_NOT_READY;
     if (latency_ms) *latency_ms = r.t_done - r.t_submit;
@@ -349,6 +349,42 @@ int pcr_server_wait(pcr_server* srv, uint64_t ticket, uint32_t* rgba, double* la
     }
 }
 
+int pcr_server_wait_pinned(pcr_server* srv, uint64_t ticket, const uint32_t** rgba_pinned, double* latency_ms) {
+    PCR_CHECK(srv && rgba_pinned, "server_wait_pinned: null argument");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    for (;;) {
+        auto it = srv->reqs.find(ticket);
+        PCR_CHECK(it != srv->reqs.end(), "server_wait_pinned: unknown ticket %llu", (unsigned long long)ticket);
+        Req& r = it->second;
+        if (r.state == kCancelled) return PCR_E_CANCELLED;
+        PCR_CHECK(r.state != kTaking && r.state != kHeld, "server_wait_pinned: ticket %llu is already taken",
+                  (unsigned long long)ticket);
+        if (r.state == kDone) {
+            r.state = kHeld;
+            *rgba_pinned = reinterpret_cast<const uint32_t*>(srv->h_arena + r.off / 3);
+            if (latency_ms) *latency_ms = r.t_done - r.t_submit;
+            return PCR_OK;
+        }
+        if (srv->worker_error) {
+            set_error("server_wait_pinned: the server stopped after a HIP error");
+            return PCR_E_HIP;
+        }
+        srv->done_cv.wait_for(lk, std::chrono::milliseconds(2));
+    }
+}
+
+int pcr_server_release(pcr_server* srv, uint64_t ticket) {
+    PCR_CHECK(srv, "server_release: null server");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(ticket);
+    PCR_CHECK(it != srv->reqs.end() && it->second.state == kHeld, "server_release: ticket %llu is not held",
+              (unsigned long long)ticket);
+    srv->arena.release(it->second.off, it->second.bytes);
+    srv->reqs.erase(it);
+    srv->work_cv.notify_one();
+    return PCR_OK;
+}
+
 int pcr_server_cancel(pcr_server* srv, uint64_t ticket) {
     PCR_CHECK(srv, "server_cancel: null server");
     std::unique_lock<std::mutex> lk(srv->mu);
@@ -361,7 +397,7 @@ int pcr_server_cancel(pcr_server* srv, uint64_t ticket) {
         srv->reqs.erase(it);
     } else if (r.state == kInFlight) {
         r.state = kCancelled;            // released when its launch retires
-    } else if (r.state == kDone) {
+    } else if (r.state == kDone || r.state == kHeld) {
         srv->arena.release(r.off, r.bytes);
         srv->reqs.erase(it);
     }   // kTaking: another call is copying the finished image out; nothing to cancel

@@ -54,8 +54,9 @@ extern "C" {
  * LasLoaderSparse.cpp:110-121) and the node variant of
  * compute_loop_nodes/render.cs:26-43 (`pointOffset`@32, `level`@36).
  * min/max are the float AABB of the batch's points in cloud coordinates
- * (relative to the cloud origin). first_point must be a multiple of 4 so that
- * the position planes can be read with 128-bit loads.
+ * (relative to the cloud origin). first_point may be any index (LAS files
+ * concatenated in one cloud start at arbitrary offsets); the kernels read
+ * 128-bit quads from the aligned word below it.
  */
 typedef struct pcr_batch {
     int32_t  state;        /* @0  unused, 0                                   */
@@ -115,7 +116,11 @@ typedef struct pcr_render_params {
     int32_t  tile_px;           /* LAS2/NODES: batches whose screen rectangle fits
                                    in this many pixels merge in an LDS tile first
                                    (0 = off, else 1024/2048/4096); same result   */
-    int32_t  pad[3];
+    uint32_t batch_begin;       /* draw batches [batch_begin, batch_end) only;     */
+    uint32_t batch_end;         /* 0, 0 = all. One LAS file of a multi-file cloud
+                                   is drawn with its own view (compute_loop_las.h:
+                                   141-199 gives each file its own transform)     */
+    int32_t  pad;
 } pcr_render_params;
 
 typedef struct pcr_stats {
@@ -231,6 +236,47 @@ int pcr_cloud_download(pcr_cloud* cloud, pcr_batch* batches_host, uint32_t* lo_h
                        uint32_t* med_host, uint32_t* hi_host, uint32_t* rgba_host,
                        float* xyz16_host, uint32_t* rgba16_host);
 int pcr_cloud_destroy(pcr_cloud* cloud);
+
+/* ---- LAS ingest -------------------------------------------------------- */
+/* Header fields the reference reads (LasLoaderSparse.cpp:495-526). */
+typedef struct pcr_las_info {
+    uint64_t num_points;        /* @107 (u32) before LAS 1.4, @247 (u64) from 1.4;
+                                   capped at 1e9 as the reference does           */
+    uint32_t offset_to_points;  /* @96                                            */
+    uint16_t record_length;     /* @105 bytes per point record                    */
+    uint8_t  point_format;      /* @104 % 128 (the LAZ compression bit cleared)   */
+    uint8_t  version_major;     /* @24                                            */
+    uint8_t  version_minor;     /* @25                                            */
+    uint8_t  rgb_offset;        /* R,G,B u16 offset in a record: 20 (format 2),
+                                   28 (3), 30 (7, 8), else 0 (LasLoaderSparse.cpp:
+                                   123-132 reads the first six bytes then)       */
+    uint8_t  pad[6];
+    double   scale[3];          /* @131                                           */
+    double   offset[3];         /* @155                                           */
+    double   min[3];            /* @187, @203, @219                               */
+    double   max[3];            /* @179, @195, @211                               */
+} pcr_las_info;
+
+typedef struct pcr_las_file {
+    const void* bytes;          /* the whole file: header, VLRs, point records   */
+    uint64_t    size;
+} pcr_las_file;
+
+/* Parse a LAS 1.0-1.4 header (host only; no device work). */
+int pcr_las_parse_header(const void* bytes, uint64_t size, pcr_las_info* out);
+/* One cloud from nfiles uncompressed LAS files, encoded on the device as
+ * LasLoaderSparse::loadLas does (LasLoaderSparse.cpp:37-212): file k's points
+ * follow file k-1's in the planes (the sparse offset is the running point
+ * count), its batches cover 10,240 consecutive points each, positions are
+ * X*scale + offset - header min in double, each batch AABB and 30-bit
+ * quantisation are computed in double, the header stores the float AABB,
+ * first_point = the sparse index and @36 (level) = k, and colour is
+ * R|G<<8|B<<16 with 16-bit channels v < 256 ? v : v / 256 (alpha 0).
+ * first_batch[k] receives file k's first batch (nfiles + 1 entries); draw
+ * file k with batch_begin/batch_end = first_batch[k], first_batch[k+1] and a
+ * view whose origin is file k's header min. */
+int pcr_cloud_from_las(pcr_ctx* ctx, const pcr_las_file* files, int nfiles, uint32_t* first_batch,
+                       pcr_cloud** out);
 
 /* ---- LOD octree (Potree-style nodes) ----------------------------------- */
 typedef struct pcr_octree_opts {
@@ -352,6 +398,13 @@ int pcr_server_submit(pcr_server* srv, const pcr_request* req, uint64_t* ticket)
 /* Non-blocking: PCR_OK with the image copied to rgba_host (w*h uint32), or PCR_E_NOT_READY. */
 int pcr_server_poll(pcr_server* srv, uint64_t ticket, uint32_t* rgba_host, double* latency_ms);
 int pcr_server_wait(pcr_server* srv, uint64_t ticket, uint32_t* rgba_host, double* latency_ms);
+/* Zero-copy form of pcr_server_wait (no counterpart in the reference; for
+ * clients that consume images in place): waits, then returns a pointer to
+ * the ticket's w*h RGBA8 image in the server's pinned result memory. The
+ * image stays valid, and its framebuffer memory stays counted against the
+ * budget, until pcr_server_release(ticket) (or pcr_server_cancel). */
+int pcr_server_wait_pinned(pcr_server* srv, uint64_t ticket, const uint32_t** rgba_pinned, double* latency_ms);
+int pcr_server_release(pcr_server* srv, uint64_t ticket);
 int pcr_server_cancel(pcr_server* srv, uint64_t ticket);
 int pcr_server_stats_get(pcr_server* srv, pcr_server_stats* out);
 int pcr_server_destroy(pcr_server* srv);

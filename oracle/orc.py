@@ -13,7 +13,7 @@ import subprocess
 
 import numpy as np
 
-from compute_rasterizer_amd.capi import BATCH_DTYPE, FB_CLEAR, GenCfg, View
+from compute_rasterizer_amd.capi import BATCH_DTYPE, FB_CLEAR, GenCfg, LasInfo, View
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "liborc.so")
@@ -61,6 +61,10 @@ def load():
         f.orc_lod_select.argtypes = [P, C.c_uint32, C.POINTER(View), C.c_int, C.c_int, C.c_float, P]
         f.orc_render_nodes.argtypes = [P, C.c_uint32, P, P, P, C.POINTER(View), C.c_int, C.c_int, C.c_float, P, P]
         f.orc_render_batched_mt.argtypes = [P, C.c_uint32, P, P, P, C.POINTER(View), C.c_int, P, C.c_int]
+        f.orc_las_parse_header.restype = C.c_int
+        f.orc_las_parse_header.argtypes = [P, C.c_uint64, C.POINTER(LasInfo)]
+        f.orc_las_encode.argtypes = [P, C.c_uint64, C.POINTER(LasInfo), C.c_uint32, C.c_uint64, C.c_uint32,
+                                     P, P, P, P, P]
         D3 = C.POINTER(C.c_double)
         f.orc_look_at_view.argtypes = [D3, D3, D3, C.c_double, C.c_double, C.c_double, D3, C.c_int, C.c_int,
                                        C.POINTER(View)]
@@ -218,3 +222,24 @@ def look_at(eye, target, up=(0, 1, 0), fovy=60.0, near=0.1, far=1000.0, origin=(
     d = lambda x: (C.c_double * 3)(*map(float, x))  # noqa: E731
     load().orc_look_at_view(d(eye), d(target), d(up), fovy, near, far, d(origin), width, height, C.byref(v))
     return v
+
+
+def las_parse_header(data: bytes):
+    """pcr_las_info of a LAS file's bytes, or None when the header is invalid."""
+    buf = np.frombuffer(data, np.uint8)
+    info = LasInfo()
+    rc = load().orc_las_parse_header(_p(buf), buf.size, C.byref(info))
+    return info if rc == 0 else None
+
+
+def las_encode(data: bytes, info: LasInfo, batch_size=10240, sparse_offset=0, file_index=0):
+    """(batches, lo, med, hi, rgba) of one LAS file, as LasLoaderSparse::loadLas builds them."""
+    buf = np.frombuffer(data, np.uint8)
+    n = int(info.num_points)
+    rec = np.ascontiguousarray(buf[info.offset_to_points:info.offset_to_points + n * info.record_length])
+    nb = (n + batch_size - 1) // batch_size
+    b = np.zeros(nb, BATCH_DTYPE)
+    lo, med, hi, rgba = (np.zeros(n, np.uint32) for _ in range(4))
+    load().orc_las_encode(_p(rec), n, C.byref(info), batch_size, sparse_offset, file_index, _p(b), _p(lo), _p(med),
+                          _p(hi), _p(rgba))
+    return b, lo, med, hi, rgba

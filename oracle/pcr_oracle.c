@@ -254,6 +254,83 @@ void orc_encode_batches(const float* xyz, uint64_t n, uint32_t bs, pcr_batch* ba
     }
 }
 
+/* ------------------------------------------------------------ LAS ingest */
+static uint16_t rd_u16(const uint8_t* p) { uint16_t v; memcpy(&v, p, 2); return v; }
+static uint32_t rd_u32(const uint8_t* p) { uint32_t v; memcpy(&v, p, 4); return v; }
+static uint64_t rd_u64(const uint8_t* p) { uint64_t v; memcpy(&v, p, 8); return v; }
+static double   rd_f64(const uint8_t* p) { double v; memcpy(&v, p, 8); return v; }
+
+/* LasLoaderSparse.cpp:495-526 (header fields), :123-132 (colour offset) */
+int orc_las_parse_header(const uint8_t* b, uint64_t size, pcr_las_info* out) {
+    memset(out, 0, sizeof(*out));
+    if (size < 227 || memcmp(b, "LASF", 4) != 0) return -1;
+    out->version_major = b[24];
+    out->version_minor = b[25];
+    const int v14 = !(out->version_major == 1 && out->version_minor < 4);
+    if (v14 && size < 255) return -1;
+    uint64_t n = v14 ? rd_u64(b + 247) : rd_u32(b + 107);
+    out->num_points = n < 1000000000ull ? n : 1000000000ull;
+    out->offset_to_points = rd_u32(b + 96);
+    out->point_format = (uint8_t)(b[104] % 128);
+    out->record_length = rd_u16(b + 105);
+    const int f = out->point_format;
+    out->rgb_offset = (uint8_t)(f == 2 ? 20 : f == 3 ? 28 : (f == 7 || f == 8) ? 30 : 0);
+    for (int a = 0; a < 3; a++) {
+        out->scale[a] = rd_f64(b + 131 + 8 * a);
+        out->offset[a] = rd_f64(b + 155 + 8 * a);
+        out->max[a] = rd_f64(b + 179 + 16 * a);
+        out->min[a] = rd_f64(b + 187 + 16 * a);
+    }
+    if (out->record_length < 12 || out->record_length < out->rgb_offset + 6) return -1;
+    if ((uint64_t)out->offset_to_points + out->num_points * out->record_length > size) return -1;
+    return 0;
+}
+
+/* LasLoaderSparse.cpp:37-212: positions in double relative to the header
+ * min, per-batch double AABB, float header, double 30-bit quantisation,
+ * 16-to-8-bit colour. */
+void orc_las_encode(const uint8_t* rec, uint64_t n, const pcr_las_info* info, uint32_t bs, uint64_t sparse,
+                    uint32_t file_index, pcr_batch* batches, uint32_t* lo, uint32_t* med, uint32_t* hi,
+                    uint32_t* rgba) {
+    const uint64_t nb = (n + bs - 1) / bs, L = info->record_length;
+    for (uint64_t bi = 0; bi < nb; bi++) {
+        const uint64_t first = bi * bs, cnt = (n - first) < bs ? (n - first) : bs;
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint64_t i = first; i < first + cnt; i++)
+            for (int a = 0; a < 3; a++) {
+                const double x = (double)(int32_t)rd_u32(rec + i * L + 4 * a) * info->scale[a] + info->offset[a] -
+                                 info->min[a];
+                mn[a] = x < mn[a] ? x : mn[a];
+                mx[a] = x > mx[a] ? x : mx[a];
+            }
+        pcr_batch* h = &batches[bi];
+        memset(h, 0, sizeof(*h));
+        for (int a = 0; a < 3; a++) { h->min[a] = (float)mn[a]; h->max[a] = (float)mx[a]; }
+        h->num_points = (uint32_t)cnt;
+        h->first_point = (uint32_t)(sparse + first);
+        h->level = file_index;
+        h->parent = -1;
+        const double size[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+        for (uint64_t i = first; i < first + cnt; i++) {
+            const uint8_t* r = rec + i * L;
+            uint32_t q[3];
+            for (int a = 0; a < 3; a++) {
+                const double x = (double)(int32_t)rd_u32(r + 4 * a) * info->scale[a] + info->offset[a] - info->min[a];
+                orc_quantize30(x, mn[a], size[a], &q[a]);
+            }
+            lo[i]  = ((q[0] >> 20) & 1023u) | (((q[1] >> 20) & 1023u) << 10) | (((q[2] >> 20) & 1023u) << 20);
+            med[i] = ((q[0] >> 10) & 1023u) | (((q[1] >> 10) & 1023u) << 10) | (((q[2] >> 10) & 1023u) << 20);
+            hi[i]  = (q[0] & 1023u) | ((q[1] & 1023u) << 10) | ((q[2] & 1023u) << 20);
+            uint32_t c[3];
+            for (int k = 0; k < 3; k++) {
+                const uint32_t v = rd_u16(r + info->rgb_offset + 2 * k);
+                c[k] = v < 256 ? v : v / 256;
+            }
+            rgba[i] = c[0] | (c[1] << 8) | (c[2] << 16);
+        }
+    }
+}
+
 /* ------------------------------------------------------- per-batch stages */
 static void mul_point(const float m[16], float x, float y, float z, float r[4]) {
     for (int i = 0; i < 4; i++) r[i] = fmaf(m[8 + i], z, fmaf(m[4 + i], y, fmaf(m[i], x, m[12 + i])));

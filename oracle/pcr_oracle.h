@@ -40,6 +40,15 @@ void orc_encode_batches(const float* xyz, uint64_t n, uint32_t batch_size,
                         pcr_batch* batches, uint32_t* lo, uint32_t* med, uint32_t* hi);
 void orc_quantize30(double x, double mn, double size, uint32_t* q);
 
+/* ---- LAS ingest (LasLoaderSparse.cpp:37-212, 495-526) ------------------ */
+/* 0 on success, -1 for a missing signature or a short header/record range. */
+int  orc_las_parse_header(const uint8_t* bytes, uint64_t size, pcr_las_info* out);
+/* One file's records -> batches of batch_size consecutive points; lo/med/hi/
+ * rgba point at the file's first slot, first_point = sparse_offset + i. */
+void orc_las_encode(const uint8_t* records, uint64_t n, const pcr_las_info* info, uint32_t batch_size,
+                    uint64_t sparse_offset, uint32_t file_index, pcr_batch* batches,
+                    uint32_t* lo, uint32_t* med, uint32_t* hi, uint32_t* rgba);
+
 /* ---- per-batch stages --------------------------------------------------- */
 int   orc_frustum_visible(const float tf[16], const float mn[3], const float mx[3]);
 float orc_projected_px(const float mn[3], const float mx[3], float len,

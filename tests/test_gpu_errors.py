@@ -28,9 +28,6 @@ def expect_invalid(fn, text):
 def test_upload_validation(ctx, small):
     xyz, rgba, b, lo, med, hi = small
     bad = b.copy()
-    bad[3]["first_point"] += 2
-    expect_invalid(lambda: ctx.upload_batched(bad, lo, med, hi, rgba), "multiple of 4")
-    bad = b.copy()
     bad[-1]["num_points"] += 64
     expect_invalid(lambda: ctx.upload_batched(bad, lo, med, hi, rgba), "exceeds")
     bad = b.copy()

@@ -156,3 +156,33 @@ def test_server_refuses_and_cancels(ctx, city):
     st = srv.stats()
     assert st.refused == 1 and st.completed == 5 and st.fb_bytes_in_use == 0
     srv.close()
+
+
+def test_server_pinned_results(ctx, city):
+    """Zero-copy results: the pinned image equals the copied one, stays
+    counted against the budget until released, and cannot be taken twice."""
+    srv = pcr.Server(ctx, budget_bytes=12 * 1280 * 720 * 2, max_batch=2, num_streams=2)
+    cid = srv.add_cloud(city)
+    vs = views_for([(640, 480), (1280, 720)], seed=21)
+    copied = []
+    for v in vs:
+        t = srv.submit(cid, v)
+        copied.append(srv.wait(t, v.width * v.height)[0])
+    ts = [srv.submit(cid, v) for v in vs]
+    held = []
+    for t, v in zip(ts, vs):
+        img, lat = srv.wait_pinned(t, v.width * v.height)
+        assert lat > 0
+        held.append(img)
+    assert srv.stats().fb_bytes_in_use > 0
+    for img, ref in zip(held, copied):
+        assert np.array_equal(img, ref)
+    with pytest.raises(pcr.PcrError) as e:
+        srv.wait(ts[0], vs[0].width * vs[0].height)
+    assert "already taken" in str(e.value)
+    srv.release(ts[0])
+    srv.cancel(ts[1])   # cancelling a held result releases it too
+    with pytest.raises(pcr.PcrError):
+        srv.release(ts[0])
+    assert srv.stats().fb_bytes_in_use == 0
+    srv.close()
