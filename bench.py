@@ -26,6 +26,24 @@ sys.path.insert(0, ROOT)
 
 import compute_rasterizer_amd as pcr  # noqa: E402
 
+# HBM bytes per launch of the default raster kernel on the default workload,
+# from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/
+# (a counter pass cannot run inside the timed loop)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "raster_tiled_city1e9_1080p_traffic.json")
+
+
+def measured_traffic(kernel: str, config: dict):
+    """Counter-measured HBM bytes per launch for this kernel and workload, or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    want = d.get("config", {})
+    if d.get("kernel") != kernel or any(config.get(k) != v for k, v in want.items()):
+        return None
+    return int(d["traffic_bytes_per_launch"])
+
 HBM_PEAK_GBS = 8000.0
 
 
@@ -397,6 +415,7 @@ def main():
                      "bytes_per_launch_mean": int(np.mean([raster_bytes[i] for i in idx]))},
         "generate_s": round(gen_s, 2),
     }
+    out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"], out["config"])
     if args.mv_views > 0:
         out["multiview"] = multiview_leg(ctx, cloud, info.num_points, args.mv_views, args.mv_frames, args.tile_px)
         print(f"[bench] multiview {out['multiview']}", file=sys.stderr, flush=True)

@@ -12,12 +12,13 @@ import numpy as np
 
 from . import capi
 from .capi import (BATCH_DTYPE, BG_BLACK, BG_LAS, FB_CLEAR, FB_HQS, FB_KEY64, LOD_SIZE80, LOD_SPACING,
-                   MODE_HQS, MODE_HQS16, MODE_LAS, MODE_LAS2, MODE_NODES, MODE_PLAIN16, POINTS_PER_BATCH,
+                   MODE_HQS, MODE_HQS16, MODE_LAS, MODE_LAS2, MODE_NODES, MODE_NODES_HQS, MODE_PLAIN16,
+                   POINTS_PER_BATCH,
                    SCENE_CITY, SCENE_CUBE, SCENE_TERRAIN, BuildOpts, GenCfg, OctreeOpts, PcrError,
                    RenderParams, Stats, Tune, View, check, ptr)
 
 __all__ = ["Context", "Cloud", "Framebuffer", "look_at", "params", "PcrError", "View", "RenderParams",
-           "MODE_PLAIN16", "MODE_LAS", "MODE_LAS2", "MODE_HQS", "MODE_NODES", "MODE_HQS16",
+           "MODE_PLAIN16", "MODE_LAS", "MODE_LAS2", "MODE_HQS", "MODE_NODES", "MODE_HQS16", "MODE_NODES_HQS",
            "LOD_SPACING", "LOD_SIZE80", "SCENE_CUBE", "SCENE_TERRAIN", "SCENE_CITY", "FB_KEY64", "FB_HQS",
            "BG_LAS", "BG_BLACK", "FB_CLEAR", "POINTS_PER_BATCH", "BATCH_DTYPE"]
 
@@ -32,14 +33,38 @@ def look_at(eye, target, up=(0.0, 1.0, 0.0), fovy=60.0, near=0.1, far=1000.0, or
 
 
 def params(mode=MODE_LAS2, culling=1, background=None, lod_rule=LOD_SPACING, lod_px=1.0, block_size=0,
-           clear_before=1, time_stages=0, tile_px=None, batch_begin=0, batch_end=0) -> RenderParams:
+           clear_before=1, time_stages=0, tile_px=None, batch_begin=0, batch_end=0, resolve_window=0) -> RenderParams:
     """Render parameters; LAS2 and NODES default to the LDS-tiled raster (2048-px tile)."""
     if background is None:
         background = BG_BLACK if mode in (MODE_PLAIN16,) else BG_LAS
     if tile_px is None:
         tile_px = 2048 if mode in (MODE_LAS2, MODE_NODES) else 0
     return RenderParams(mode, culling, lod_rule, lod_px, background, block_size, clear_before, time_stages,
-                        tile_px, batch_begin, batch_end)
+                        tile_px, batch_begin, batch_end, resolve_window)
+
+
+def potree_info(metadata_json: str) -> capi.PotreeInfo:
+    """pcr_potree_info from a Potree 2.0 metadata.json (PotreeData.h:144-182)."""
+    import json
+    m = json.loads(metadata_json)
+    if m.get("encoding", "DEFAULT") != "DEFAULT":
+        raise ValueError(f"Potree encoding {m.get('encoding')!r} is not read (only DEFAULT)")
+    info = capi.PotreeInfo()
+    for a in range(3):
+        info.scale[a] = m["scale"][a]
+        info.offset[a] = m["offset"][a]
+        info.min[a] = m["boundingBox"]["min"][a]
+        info.max[a] = m["boundingBox"]["max"][a]
+    info.spacing = m["spacing"]
+    info.num_points = m["points"]
+    info.first_chunk_size = m["hierarchy"]["firstChunkSize"]
+    bpp, rgb = 0, 0
+    for att in m["attributes"]:
+        if att["name"] == "rgb":
+            rgb = bpp
+        bpp += att["size"]
+    info.bytes_per_point, info.rgb_offset = bpp, rgb
+    return info
 
 
 def gen_cfg(scene, n, seed=1, extent=2000.0, amplitude=150.0, jitter=0.5, num_buildings=256,
@@ -305,6 +330,26 @@ class Context:
         o = OctreeOpts(seed, root_spacing_div, max_node_points, max_depth)
         check(capi.load().pcr_octree_generate(self.h, C.byref(cfg), C.byref(o), C.byref(h)))
         return Cloud(self, h)
+
+    def from_potree(self, metadata_json: str, hierarchy: bytes, octree: bytes) -> Cloud:
+        """LOD cloud from a Potree 2.0 dataset (metadata.json text, hierarchy.bin and octree.bin bytes)."""
+        info = potree_info(metadata_json)
+        hb = np.frombuffer(hierarchy, np.uint8)
+        ob = np.frombuffer(octree, np.uint8)
+        h = C.c_void_p()
+        check(capi.load().pcr_cloud_from_potree(self.h, C.byref(info), ptr(hb), hb.size, ptr(ob), ob.size,
+                                                C.byref(h)))
+        return Cloud(self, h)
+
+    def from_las(self, files):
+        """One cloud from LAS file bytes: (Cloud, first_batch) with file k's batches in
+        [first_batch[k], first_batch[k+1])."""
+        bufs = [np.frombuffer(d, np.uint8) for d in files]
+        arr = (capi.LasFile * len(bufs))(*[capi.LasFile(b.ctypes.data, b.size) for b in bufs])
+        first = np.zeros(len(bufs) + 1, np.uint32)
+        h = C.c_void_p()
+        check(capi.load().pcr_cloud_from_las(self.h, arr, len(bufs), ptr(first), C.byref(h)))
+        return Cloud(self, h), first
 
     def framebuffer(self, width, height, kind=FB_KEY64) -> Framebuffer:
         return Framebuffer(self, width, height, kind)

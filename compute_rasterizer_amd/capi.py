@@ -29,7 +29,7 @@ DEPTH_CLEAR = 0xFFFFFFFF
 BG_LAS = 0x00443322
 BG_BLACK = 0xFF000000
 
-MODE_PLAIN16, MODE_LAS, MODE_LAS2, MODE_HQS, MODE_NODES, MODE_HQS16 = range(6)
+MODE_PLAIN16, MODE_LAS, MODE_LAS2, MODE_HQS, MODE_NODES, MODE_HQS16, MODE_NODES_HQS = range(7)
 LOD_SPACING, LOD_SIZE80 = 0, 1
 SCENE_CUBE, SCENE_TERRAIN, SCENE_CITY = 0, 1, 2
 FB_KEY64, FB_HQS = 0, 1
@@ -60,7 +60,7 @@ class RenderParams(C.Structure):
     _fields_ = [("mode", C.c_int32), ("frustum_culling", C.c_int32), ("lod_rule", C.c_int32),
                 ("lod_px", C.c_float), ("background", C.c_uint32), ("block_size", C.c_int32),
                 ("clear_before", C.c_int32), ("time_stages", C.c_int32), ("tile_px", C.c_int32),
-                ("batch_begin", C.c_uint32), ("batch_end", C.c_uint32), ("pad", C.c_int32)]
+                ("batch_begin", C.c_uint32), ("batch_end", C.c_uint32), ("resolve_window", C.c_int32)]
 
 
 class LasInfo(C.Structure):
@@ -68,6 +68,12 @@ class LasInfo(C.Structure):
                 ("point_format", C.c_uint8), ("version_major", C.c_uint8), ("version_minor", C.c_uint8),
                 ("rgb_offset", C.c_uint8), ("pad", C.c_uint8 * 6), ("scale", C.c_double * 3),
                 ("offset", C.c_double * 3), ("min", C.c_double * 3), ("max", C.c_double * 3)]
+
+
+class PotreeInfo(C.Structure):
+    _fields_ = [("scale", C.c_double * 3), ("offset", C.c_double * 3), ("min", C.c_double * 3),
+                ("max", C.c_double * 3), ("spacing", C.c_double), ("num_points", C.c_uint64),
+                ("first_chunk_size", C.c_uint64), ("bytes_per_point", C.c_uint32), ("rgb_offset", C.c_uint32)]
 
 
 class LasFile(C.Structure):
@@ -161,6 +167,7 @@ SIGNATURES = {
     "pcr_stats_get": (C.c_int, [P, C.POINTER(Stats)]),
     "pcr_las_parse_header": (C.c_int, [P, C.c_uint64, P]),
     "pcr_cloud_from_las": (C.c_int, [P, P, C.c_int, P, C.POINTER(C.c_void_p)]),
+    "pcr_cloud_from_potree": (C.c_int, [P, P, P, C.c_uint64, P, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pcr_frame_times": (C.c_int, [P, C.c_int, P, C.POINTER(C.c_int)]),
     "pcr_selftest_divide": (C.c_int, [P, P, P, P, C.c_uint64, P]),
     "pcr_multiview_create": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(C.c_void_p),

@@ -422,14 +422,15 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
               "render: view size %dx%d differs from framebuffer %dx%d", view->width, view->height, fb->width,
               fb->height);
     const int mode = params->mode;
-    const bool hqs = mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16;
+    const bool hqs = mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16 || mode == PCR_MODE_NODES_HQS;
     const bool pts16 = mode == PCR_MODE_PLAIN16 || mode == PCR_MODE_HQS16;
-    PCR_CHECK(mode >= PCR_MODE_PLAIN16 && mode <= PCR_MODE_HQS16, "render: unknown mode %d", mode);
+    const bool lod = mode == PCR_MODE_NODES || mode == PCR_MODE_NODES_HQS;
+    PCR_CHECK(mode >= PCR_MODE_PLAIN16 && mode <= PCR_MODE_NODES_HQS, "render: unknown mode %d", mode);
     PCR_CHECK(hqs == (fb->kind == PCR_FB_HQS), "render: mode %d needs a %s framebuffer", mode,
               hqs ? "HQS" : "KEY64");
     PCR_CHECK(pts16 ? cloud->d_points16 != nullptr : cloud->d_batches != nullptr,
               "render: cloud has no %s layout for mode %d", pts16 ? "16 B/pt" : "batched", mode);
-    PCR_CHECK(mode != PCR_MODE_NODES || params->lod_rule == PCR_LOD_SPACING || params->lod_rule == PCR_LOD_SIZE80,
+    PCR_CHECK(!lod || params->lod_rule == PCR_LOD_SPACING || params->lod_rule == PCR_LOD_SIZE80,
               "render: bad lod_rule %d", params->lod_rule);
     PCR_CHECK(params->tile_px == 0 || params->tile_px == 1024 || params->tile_px == 2048 || params->tile_px == 4096,
               "render: tile_px %d not in {0,1024,2048,4096}", params->tile_px);
@@ -453,8 +454,7 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
     }
     if (ctx->timed) PCR_HIP(hipEventRecord(ev[1], s));
     if (!pts16) {
-        rc = launch_select(s, cloud, *view, *params, ctx->d_state, mode == PCR_MODE_NODES ? 80.0f : 100.0f,
-                           mode == PCR_MODE_NODES);
+        rc = launch_select(s, cloud, *view, *params, ctx->d_state, lod ? 80.0f : 100.0f, lod);
         if (rc) return rc;
     }
     ctx->last_nb = pts16 ? 0 : cloud->num_batches;
@@ -467,7 +467,8 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
             rc = params->tile_px ? launch_raster_tiled(s, cloud, *view, params->tile_px, block, ctx->d_state, fb->d_keys)
                                  : launch_raster_batched(s, cloud, *view, 4, block ? block : 256, ctx->d_state, fb->d_keys);
             break;
-        case PCR_MODE_HQS: rc = launch_hqs_batched(s, cloud, *view, ctx->d_state, fb); break;
+        case PCR_MODE_HQS:
+        case PCR_MODE_NODES_HQS: rc = launch_hqs_batched(s, cloud, *view, ctx->d_state, fb); break;
         case PCR_MODE_HQS16: rc = launch_hqs_points16(s, cloud, *view, fb); break;
     }
     if (rc) return rc;
@@ -478,12 +479,14 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
 int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params, pcr_fb* fb) {
     PCR_CHECK(ctx && cloud && params && fb, "resolve: null argument");
     const int mode = params->mode;
-    const bool hqs = mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16;
+    const bool hqs = mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16 || mode == PCR_MODE_NODES_HQS;
     PCR_CHECK(hqs == (fb->kind == PCR_FB_HQS), "resolve: mode %d does not match the framebuffer kind", mode);
+    PCR_CHECK(params->resolve_window >= 0 && params->resolve_window <= 4, "resolve: resolve_window %d not in 0..4",
+              params->resolve_window);
     PCR_CHECK(hqs || mode == PCR_MODE_PLAIN16 || cloud->d_rgba, "resolve: cloud has no colour plane");
     hipEvent_t* ev = ctx->ring[(ctx->frames_timed + PCR_TIMING_RING - 1) % PCR_TIMING_RING];
     if (ctx->timed) PCR_HIP(hipEventRecord(ev[4], ctx->stream));
-    int rc = launch_resolve(ctx->stream, cloud, mode, params->background, fb);
+    int rc = launch_resolve(ctx->stream, cloud, mode, params->background, params->resolve_window, fb);
     if (rc) return rc;
     if (ctx->timed) PCR_HIP(hipEventRecord(ev[5], ctx->stream));
     return PCR_OK;

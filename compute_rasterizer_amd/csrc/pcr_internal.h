@@ -94,7 +94,7 @@ int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, in
 int launch_raster_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb);
 int launch_hqs_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb);
 int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pcr_fb* fb);
-int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, pcr_fb* fb);
+int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, pcr_fb* fb);
 int launch_selftest_divide(hipStream_t s, const float* nx, const float* ny, const float* den, uint64_t n,
                            float* out2);
 

@@ -725,10 +725,54 @@ __global__ void k_resolve_hqs(const u32x4* __restrict__ acc, uint64_t npix, uint
     }
 }
 
-int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, pcr_fb* fb) {
+// compute_loop_nodes_hqs/resolve.cs:92-158: a float average over the
+// (2*window+1)^2 neighbourhood, weights 100 (centre), 2 (ox <= 1 && oy <= 1)
+// or 1, dropping neighbours deeper than 1.01 x the window's smallest depth.
+// Neighbours outside the image are skipped. Sums run ox-major in the
+// shader's order, so the float results are reproducible.
+__global__ void k_resolve_hqs_window(const uint32_t* __restrict__ depth, const u32x4* __restrict__ acc, int w, int h,
+                                     int window, uint32_t* __restrict__ out) {
+    const uint64_t npix = (uint64_t)w * h;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
+        const int x = (int)(i % (uint64_t)w), y = (int)(i / (uint64_t)w);
+        float dmin = 1000000.0f;
+        for (int ox = -window; ox <= window; ox++)
+            for (int oy = -window; oy <= window; oy++) {
+                const int px = x + ox, py = y + oy;
+                if (px < 0 || py < 0 || px >= w || py >= h) continue;
+                const float pd = __uint_as_float(depth[(uint64_t)py * w + px]);
+                if (pd >= 0.0f) dmin = pd < dmin ? pd : dmin;
+            }
+        float R = 0.0f, G = 0.0f, B = 0.0f, C = 0.0f;
+        for (int ox = -window; ox <= window; ox++)
+            for (int oy = -window; oy <= window; oy++) {
+                const int px = x + ox, py = y + oy;
+                if (px < 0 || py < 0 || px >= w || py >= h) continue;
+                const uint64_t q = (uint64_t)py * w + px;
+                const float pd = __uint_as_float(depth[q]);
+                float wt = (ox == 0 && oy == 0) ? 100.0f : (ox <= 1 && oy <= 1) ? 2.0f : 1.0f;
+                if (pd > dmin * 1.01f) wt = 0.0f;
+                const u32x4 a = acc[q];
+                R += (float)a.x * wt;
+                G += (float)a.y * wt;
+                B += (float)a.z * wt;
+                C += (float)a.w * wt;
+            }
+        uint32_t color = 0u;
+        if (C != 0.0f) color = (uint32_t)(R / C) | ((uint32_t)(G / C) << 8) | ((uint32_t)(B / C) << 16);
+        if (dmin < 0.0f) color = 0x0000FF00u;
+        out[i] = color;
+    }
+}
+
+int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, pcr_fb* fb) {
     const uint64_t npix = (uint64_t)fb->width * fb->height;
     const int g = grid_for(npix, 256);
-    if (mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16) {
+    if (mode == PCR_MODE_NODES_HQS) {
+        k_resolve_hqs_window<<<g, 256, 0, s>>>(fb->d_depth, reinterpret_cast<const u32x4*>(fb->d_acc), fb->width,
+                                               fb->height, window, fb->d_rgba);
+    } else if (mode == PCR_MODE_HQS || mode == PCR_MODE_HQS16) {
         k_resolve_hqs<<<g, 256, 0, s>>>(reinterpret_cast<const u32x4*>(fb->d_acc), npix, bg, fb->d_rgba);
     } else if (mode == PCR_MODE_PLAIN16) {
         k_resolve_color<<<g, 256, 0, s>>>(fb->d_keys, npix, bg, fb->d_rgba);

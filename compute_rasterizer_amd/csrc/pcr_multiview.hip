@@ -132,8 +132,12 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster(const pcr_batch* __restrict
         __syncthreads();
         const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
         if (item >= total) break;
-        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[item / chunks_per_batch]);
-        const uint32_t chunk = item % chunks_per_batch;
+        // spread order, as in the single-view tiled raster: consecutive items
+        // come from distant parts of the list (2654435761 is a prime above
+        // any item count, so the map is a permutation)
+        const uint32_t sp = (uint32_t)(((uint64_t)item * 2654435761ull) % total);
+        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[sp / chunks_per_batch]);
+        const uint32_t chunk = sp % chunks_per_batch;
         // compact the views that draw this batch (wave 0, one lane per view)
         if (threadIdx.x < 64) {
             const int v = (int)threadIdx.x;

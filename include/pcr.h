@@ -95,7 +95,9 @@ typedef enum pcr_mode {
     PCR_MODE_LAS2     = 2,  /* same, 4-consecutive-point vector loads  (compute_loop_las2)  */
     PCR_MODE_HQS      = 3,  /* two-pass depth + accumulate            (compute_loop_las_hqs)*/
     PCR_MODE_NODES    = 4,  /* LOD octree nodes                       (compute_loop_nodes)  */
-    PCR_MODE_HQS16    = 5   /* hqs over the 16 B/pt layout                                 */
+    PCR_MODE_HQS16    = 5,  /* hqs over the 16 B/pt layout                                 */
+    PCR_MODE_NODES_HQS = 6  /* hqs over the LOD node cut, window-weighted resolve
+                               (compute_loop_nodes_hqs)                                    */
 } pcr_mode;
 
 /* LOD criteria for PCR_MODE_NODES */
@@ -120,7 +122,8 @@ typedef struct pcr_render_params {
     uint32_t batch_end;         /* 0, 0 = all. One LAS file of a multi-file cloud
                                    is drawn with its own view (compute_loop_las.h:
                                    141-199 gives each file its own transform)     */
-    int32_t  pad;
+    int32_t  resolve_window;    /* PCR_MODE_NODES_HQS: neighbourhood radius of the
+                                   resolve, 0..4 (0 in nodes_hqs/resolve.cs:98)    */
 } pcr_render_params;
 
 typedef struct pcr_stats {
@@ -277,6 +280,34 @@ int pcr_las_parse_header(const void* bytes, uint64_t size, pcr_las_info* out);
  * view whose origin is file k's header min. */
 int pcr_cloud_from_las(pcr_ctx* ctx, const pcr_las_file* files, int nfiles, uint32_t* first_batch,
                        pcr_cloud** out);
+
+/* ---- Potree 2.0 ingest -------------------------------------------------- */
+/* The metadata.json fields the reader uses (PotreeData.h:144-182); the caller
+ * parses the JSON. Only the "DEFAULT" (uncompressed) encoding is read. */
+typedef struct pcr_potree_info {
+    double   scale[3];          /* "scale"                                        */
+    double   offset[3];         /* "offset"                                       */
+    double   min[3];            /* "boundingBox"."min"                            */
+    double   max[3];            /* "boundingBox"."max"                            */
+    double   spacing;           /* "spacing" (root node)                          */
+    uint64_t num_points;        /* "points"                                       */
+    uint64_t first_chunk_size;  /* "hierarchy"."firstChunkSize"                   */
+    uint32_t bytes_per_point;   /* sum of the attribute sizes                     */
+    uint32_t rgb_offset;        /* byte offset of the "rgb" attribute             */
+} pcr_potree_info;
+
+/* One LOD cloud from a Potree 2.0 hierarchy.bin and octree.bin (host bytes).
+ * The hierarchy is read as PotreeData::parseHierarchy does
+ * (PotreeData.h:185-248: 22-byte node records, proxy nodes of type 2 pointing
+ * at further chunks, child boxes halved in float). Nodes become batches in
+ * octree.bin order (PotreeData.h:276-278), with first_point = byteOffset /
+ * bytes_per_point, @36 = depth, parent = the parent's batch index and spacing =
+ * root spacing / 2^depth. Points are encoded on the device against their
+ * node's box (PotreeData.h:439-511), with the clamps of
+ * LasLoaderSparse.cpp:150-152 in place of the reference's 30-bit wrap, and
+ * the 10-bit fine plane from bits 9..0. */
+int pcr_cloud_from_potree(pcr_ctx* ctx, const pcr_potree_info* info, const void* hierarchy,
+                          uint64_t hierarchy_size, const void* octree, uint64_t octree_size, pcr_cloud** out);
 
 /* ---- LOD octree (Potree-style nodes) ----------------------------------- */
 typedef struct pcr_octree_opts {

@@ -11,8 +11,11 @@
 // pcr_host::Error (the reference skipped silently, compute_loop_las.h:123,210).
 #pragma once
 #include <cstdint>
+#include <fstream>
+#include <iterator>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 #include "pcr.h"
 
@@ -60,14 +63,74 @@ struct Runtime {
     static Method* getSelectedMethod() { return selectedMethod; }
 };
 
+// One drawable piece of a cloud: a batch range (0, 0 = all) and the origin
+// its coordinates are relative to. The reference gives every LAS file its
+// own translate(boxMin) (compute_loop_las.h:170-176).
+struct CloudPart {
+    uint32_t batch_begin = 0, batch_end = 0;
+    double origin[3] = {0.0, 0.0, 0.0};
+};
+
+inline std::vector<char> read_file(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw Error(PCR_E_INVALID, "cannot open " + path);
+    return std::vector<char>(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+}
+
+// LasLoaderSparse's job (LasLoaderSparse.cpp:37-212, 495-526): reads the
+// files, encodes them into one cloud on the device, one part per file.
+struct LasFiles {
+    pcr_cloud* cloud = nullptr;
+    std::vector<CloudPart> parts;
+    LasFiles(pcr_ctx* ctx, const std::vector<std::string>& paths) {
+        std::vector<std::vector<char>> data;
+        std::vector<pcr_las_file> files;
+        for (const std::string& p : paths) data.push_back(read_file(p));
+        for (const auto& d : data) files.push_back(pcr_las_file{d.data(), (uint64_t)d.size()});
+        std::vector<uint32_t> first(files.size() + 1);
+        check(pcr_cloud_from_las(ctx, files.data(), (int)files.size(), first.data(), &cloud));
+        for (size_t k = 0; k < files.size(); k++) {
+            pcr_las_info info;
+            check(pcr_las_parse_header(files[k].bytes, files[k].size, &info));
+            CloudPart part;
+            part.batch_begin = first[k];
+            part.batch_end = first[k + 1];
+            for (int a = 0; a < 3; a++) part.origin[a] = info.min[a];
+            parts.push_back(part);
+        }
+    }
+    ~LasFiles() { pcr_cloud_destroy(cloud); }
+    LasFiles(const LasFiles&) = delete;
+    LasFiles& operator=(const LasFiles&) = delete;
+};
+
+// PotreeData::create + load (PotreeData.h:144-511) for a dataset whose
+// metadata fields the caller has read from metadata.json.
+struct PotreeCloud {
+    pcr_cloud* cloud = nullptr;
+    CloudPart part;
+    PotreeCloud(pcr_ctx* ctx, const pcr_potree_info& info, const std::string& dir) {
+        const std::vector<char> h = read_file(dir + "/hierarchy.bin"), o = read_file(dir + "/octree.bin");
+        check(pcr_cloud_from_potree(ctx, &info, h.data(), h.size(), o.data(), o.size(), &cloud));
+        for (int a = 0; a < 3; a++) part.origin[a] = info.min[a];
+    }
+    ~PotreeCloud() { pcr_cloud_destroy(cloud); }
+    PotreeCloud(const PotreeCloud&) = delete;
+    PotreeCloud& operator=(const PotreeCloud&) = delete;
+};
+
 // Shared body of the batched methods: one framebuffer per method, resized
-// when the view size changes (compute_loop_las.h:130-138), render + resolve
-// + readback of the image each frame.
+// when the view size changes (compute_loop_las.h:130-138), render of every
+// part (the framebuffer is cleared before the first) + resolve + readback of
+// the image each frame.
 class BatchedMethod : public Method {
   public:
-    BatchedMethod(pcr_ctx* ctx, pcr_cloud* cloud, const double origin[3], int mode, int fb_kind)
-        : ctx_(ctx), cloud_(cloud), mode_(mode), fb_kind_(fb_kind) {
-        for (int i = 0; i < 3; i++) origin_[i] = origin ? origin[i] : 0.0;
+    BatchedMethod(pcr_ctx* ctx, pcr_cloud* cloud, std::vector<CloudPart> parts, int mode, int fb_kind)
+        : ctx_(ctx), cloud_(cloud), parts_(std::move(parts)), mode_(mode), fb_kind_(fb_kind) {
+        // hqs runs its depth and colour passes per render call, so several
+        // parts would colour before the last part's depth is known
+        if ((mode == PCR_MODE_HQS || mode == PCR_MODE_NODES_HQS) && parts_.size() > 1)
+            throw Error(PCR_E_INVALID, "the hqs methods draw one part per frame");
         params_ = pcr_render_params{};
         params_.mode = mode;
         params_.frustum_culling = 1;
@@ -88,9 +151,15 @@ class BatchedMethod : public Method {
             fb_w_ = v.width;
             fb_h_ = v.height;
         }
-        pcr_view pv;
-        check(pcr_view_from_matrices(v.view, v.proj, origin_, v.width, v.height, &pv));
-        check(pcr_render(ctx_, cloud_, &pv, &params_, fb_));
+        for (size_t k = 0; k < parts_.size(); k++) {
+            pcr_view pv;
+            check(pcr_view_from_matrices(v.view, v.proj, parts_[k].origin, v.width, v.height, &pv));
+            pcr_render_params p = params_;
+            p.clear_before = k == 0 ? params_.clear_before : 0;
+            p.batch_begin = parts_[k].batch_begin;
+            p.batch_end = parts_[k].batch_end;
+            check(pcr_render(ctx_, cloud_, &pv, &p, fb_));
+        }
         check(pcr_resolve(ctx_, cloud_, &params_, fb_));
         v.image.resize((size_t)v.width * v.height);
         check(pcr_fb_readback_rgba(ctx_, fb_, v.image.data()));
@@ -101,38 +170,60 @@ class BatchedMethod : public Method {
   protected:
     pcr_ctx* ctx_;
     pcr_cloud* cloud_;
-    double origin_[3];
+    std::vector<CloudPart> parts_;
     int mode_, fb_kind_;
     pcr_render_params params_;
     pcr_fb* fb_ = nullptr;
     int fb_w_ = 0, fb_h_ = 0;
 };
 
+inline std::vector<CloudPart> one_part(const double origin[3]) {
+    std::vector<CloudPart> parts(1);
+    for (int i = 0; i < 3; i++) parts[0].origin[i] = origin ? origin[i] : 0.0;
+    return parts;
+}
+
 struct ComputeLoopLas : BatchedMethod {       // compute_loop_las.h:80-92
-    ComputeLoopLas(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : BatchedMethod(c, cl, o, PCR_MODE_LAS, PCR_FB_KEY64) {
+    ComputeLoopLas(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : ComputeLoopLas(c, cl, one_part(o)) {}
+    ComputeLoopLas(pcr_ctx* c, pcr_cloud* cl, std::vector<CloudPart> parts)
+        : BatchedMethod(c, cl, std::move(parts), PCR_MODE_LAS, PCR_FB_KEY64) {
         name = "loop_las";
         description = "one workgroup per batch, frustum cull, adaptive 10/20/30-bit precision";
         group = "10-10-10 bit encoded";
     }
 };
 struct ComputeLoopLas2 : BatchedMethod {      // compute_loop_las2.h:80-94
-    ComputeLoopLas2(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : BatchedMethod(c, cl, o, PCR_MODE_LAS2, PCR_FB_KEY64) {
+    ComputeLoopLas2(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : ComputeLoopLas2(c, cl, one_part(o)) {}
+    ComputeLoopLas2(pcr_ctx* c, pcr_cloud* cl, std::vector<CloudPart> parts)
+        : BatchedMethod(c, cl, std::move(parts), PCR_MODE_LAS2, PCR_FB_KEY64) {
         name = "loop_las_prefetch";
         description = "as loop_las, four consecutive points per thread with 128-bit loads";
         group = "10-10-10 bit encoded";
     }
 };
 struct ComputeLoopLasHqs : BatchedMethod {    // compute_loop_las_hqs.h:86-95
-    ComputeLoopLasHqs(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : BatchedMethod(c, cl, o, PCR_MODE_HQS, PCR_FB_HQS) {
+    ComputeLoopLasHqs(pcr_ctx* c, pcr_cloud* cl, const double o[3])
+        : BatchedMethod(c, cl, one_part(o), PCR_MODE_HQS, PCR_FB_HQS) {
         name = "loop_las_hqs";
         description = "high-quality shading: depth pass, then averaging of points within 1% of the depth";
         group = "10-10-10 bit encoded";
     }
 };
 struct ComputeLoopNodes : BatchedMethod {     // compute_loop_nodes.h:74-84
-    ComputeLoopNodes(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : BatchedMethod(c, cl, o, PCR_MODE_NODES, PCR_FB_KEY64) {
+    ComputeLoopNodes(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : ComputeLoopNodes(c, cl, one_part(o)) {}
+    ComputeLoopNodes(pcr_ctx* c, pcr_cloud* cl, std::vector<CloudPart> parts)
+        : BatchedMethod(c, cl, std::move(parts), PCR_MODE_NODES, PCR_FB_KEY64) {
         name = "loop_nodes";
         description = "LOD octree nodes: frustum and screen-space error cut";
+        group = "render Potree nodes";
+    }
+};
+
+struct ComputeLoopNodesHqs : BatchedMethod {  // compute_loop_nodes_hqs.h:82-92
+    ComputeLoopNodesHqs(pcr_ctx* c, pcr_cloud* cl, const double o[3])
+        : BatchedMethod(c, cl, one_part(o), PCR_MODE_NODES_HQS, PCR_FB_HQS) {
+        name = "loop_nodes_hqs";
+        description = "LOD octree nodes with high-quality shading and a weighted resolve";
         group = "render Potree nodes";
     }
 };

@@ -13,7 +13,7 @@ import subprocess
 
 import numpy as np
 
-from compute_rasterizer_amd.capi import BATCH_DTYPE, FB_CLEAR, GenCfg, LasInfo, View
+from compute_rasterizer_amd.capi import BATCH_DTYPE, FB_CLEAR, GenCfg, LasInfo, PotreeInfo, View
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "liborc.so")
@@ -65,6 +65,11 @@ def load():
         f.orc_las_parse_header.argtypes = [P, C.c_uint64, C.POINTER(LasInfo)]
         f.orc_las_encode.argtypes = [P, C.c_uint64, C.POINTER(LasInfo), C.c_uint32, C.c_uint64, C.c_uint32,
                                      P, P, P, P, P]
+        f.orc_hqs_nodes.argtypes = [P, C.c_uint32, P, P, P, P, C.POINTER(View), C.c_int, C.c_int, C.c_float, P, P]
+        f.orc_hqs_resolve_window.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P]
+        f.orc_potree_nodes.restype = C.c_int64
+        f.orc_potree_nodes.argtypes = [C.POINTER(PotreeInfo), P, C.c_uint64, P, P, C.c_uint64]
+        f.orc_potree_encode.argtypes = [C.POINTER(PotreeInfo), P, P, P, C.c_uint64, P, P, P, P]
         D3 = C.POINTER(C.c_double)
         f.orc_look_at_view.argtypes = [D3, D3, D3, C.c_double, C.c_double, C.c_double, D3, C.c_int, C.c_int,
                                        C.POINTER(View)]
@@ -243,3 +248,41 @@ def las_encode(data: bytes, info: LasInfo, batch_size=10240, sparse_offset=0, fi
     load().orc_las_encode(_p(rec), n, C.byref(info), batch_size, sparse_offset, file_index, _p(b), _p(lo), _p(med),
                           _p(hi), _p(rgba))
     return b, lo, med, hi, rgba
+
+
+def potree_nodes(info: PotreeInfo, hierarchy: bytes, max_nodes=1 << 22):
+    """(batches, boxes) of a Potree 2.0 hierarchy in octree.bin order, or None when it is malformed."""
+    h = np.frombuffer(hierarchy, np.uint8)
+    b = np.zeros(max_nodes, BATCH_DTYPE)
+    boxes = np.zeros((max_nodes, 6), np.float64)
+    n = load().orc_potree_nodes(C.byref(info), _p(h), h.size, _p(b), _p(boxes), max_nodes)
+    if n < 0:
+        return None
+    return b[:n].copy(), boxes[:n].copy()
+
+
+def potree_encode(info: PotreeInfo, octree: bytes, b, boxes):
+    """(lo, med, hi, rgba) of a Potree 2.0 octree.bin."""
+    o = np.frombuffer(octree, np.uint8)
+    n = int(info.num_points)
+    lo, med, hi, rgba = (np.zeros(n, np.uint32) for _ in range(4))
+    load().orc_potree_encode(C.byref(info), _p(o), _p(b), _p(np.ascontiguousarray(boxes)), len(b), _p(lo), _p(med),
+                             _p(hi), _p(rgba))
+    return lo, med, hi, rgba
+
+
+def hqs_nodes(nodes, lo, med, hi, rgba, view: View, culling=1, rule=0, lod_px=1.0):
+    """(depth, acc) of the hqs passes over the LOD node cut."""
+    npix = view.width * view.height
+    depth = np.full(npix, 0xFFFFFFFF, np.uint32)
+    acc = np.zeros(4 * npix, np.uint32)
+    load().orc_hqs_nodes(_p(nodes), len(nodes), _p(lo), _p(med), _p(hi), _p(rgba), C.byref(view), culling, rule,
+                         lod_px, _p(depth), _p(acc))
+    return depth, acc
+
+
+def hqs_resolve_window(depth, acc, width, height, window=0):
+    out = np.zeros(width * height, np.uint32)
+    load().orc_hqs_resolve_window(_p(np.ascontiguousarray(depth, np.uint32)), _p(np.ascontiguousarray(acc, np.uint32)),
+                                  width, height, window, _p(out))
+    return out

@@ -224,11 +224,15 @@ void orc_morton_order(const float* xyz, uint64_t n, uint32_t* perm, uint64_t* ke
 }
 
 /* ---------------------------------------------------------------- encode */
-/* LasLoaderSparse.cpp:146-152 with the zero-extent decision */
+/* LasLoaderSparse.cpp:146-152 with the zero-extent decision; a position
+ * below the box (possible against a Potree node's float box) clamps to 0
+ * instead of converting a negative double to uint32 */
 void orc_quantize30(double x, double mn, double size, uint32_t* q) {
     if (size == 0.0) { *q = 0; return; }
-    uint32_t v = (uint32_t)(((x - mn) / size) * 1073741824.0);
-    *q = v < 1073741823u ? v : 1073741823u;
+    const double f = ((x - mn) / size) * 1073741824.0;
+    if (!(f > 0.0)) { *q = 0; return; }
+    uint32_t v = f < 1073741823.0 ? (uint32_t)f : 1073741823u;
+    *q = v;
 }
 void orc_encode_batches(const float* xyz, uint64_t n, uint32_t bs, pcr_batch* batches,
                         uint32_t* lo, uint32_t* med, uint32_t* hi) {
@@ -325,6 +329,173 @@ void orc_las_encode(const uint8_t* rec, uint64_t n, const pcr_las_info* info, ui
             for (int k = 0; k < 3; k++) {
                 const uint32_t v = rd_u16(r + info->rgb_offset + 2 * k);
                 c[k] = v < 256 ? v : v / 256;
+            }
+            rgba[i] = c[0] | (c[1] << 8) | (c[2] << 16);
+        }
+    }
+}
+
+/* ---------------------------------------------------------- Potree 2.0 */
+typedef struct {
+    double box[6];
+    float spacing;
+    int level, type, mask, parent, child[8];
+    uint32_t num_points;
+    int64_t byte_offset, byte_size, hier_offset, hier_size;
+    uint64_t dfs;
+} potree_node;
+
+typedef struct {
+    potree_node* n;
+    uint64_t count, cap;
+} potree_pool;
+
+static int pool_add(potree_pool* p) {
+    if (p->count == p->cap) {
+        p->cap = p->cap ? 2 * p->cap : 256;
+        p->n = (potree_node*)realloc(p->n, p->cap * sizeof(potree_node));
+    }
+    memset(&p->n[p->count], 0, sizeof(potree_node));
+    for (int k = 0; k < 8; k++) p->n[p->count].child[k] = -1;
+    p->n[p->count].parent = -1;
+    return (int)p->count++;
+}
+
+/* PotreeData.h:115-139: child box halved in float, bit 0 = z, 1 = y, 2 = x */
+static void child_box(const double* b, int index, double* out) {
+    float mn[3] = {(float)b[0], (float)b[1], (float)b[2]}, mx[3] = {(float)b[3], (float)b[4], (float)b[5]};
+    float size[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+    for (int bit = 0; bit < 3; bit++) {
+        const int a = 2 - bit;
+        if (index & (1 << bit)) mn[a] += size[a] / 2.0f; else mx[a] -= size[a] / 2.0f;
+    }
+    for (int a = 0; a < 3; a++) { out[a] = mn[a]; out[3 + a] = mx[a]; }
+}
+
+/* PotreeData.h:185-248: one hierarchy chunk, whose first record is `root` */
+static int parse_chunk(potree_pool* p, int root, const uint8_t* h, uint64_t hsize) {
+    const int64_t off = p->n[root].hier_offset, size = p->n[root].hier_size;
+    if (off < 0 || size < 0 || size % 22 || (uint64_t)(off + size) > hsize || size == 0) return -1;
+    const int64_t nrec = size / 22;
+    int* list = (int*)malloc(sizeof(int) * nrec);
+    int64_t fill = 1;
+    list[0] = root;
+    int rc = 0;
+    for (int64_t i = 0; i < nrec && rc == 0; i++) {
+        if (i >= fill) { rc = -1; break; }
+        potree_node* c = &p->n[list[i]];
+        const uint8_t* r = h + off + 22 * i;
+        const int type = r[0], mask = r[1];
+        const uint32_t np = rd_u32(r + 2);
+        const int64_t bo = (int64_t)rd_u64(r + 6), bsz = (int64_t)rd_u64(r + 14);
+        if (c->type == 2) { c->byte_offset = bo; c->byte_size = bsz; c->num_points = np; }
+        else if (type == 2) { c->hier_offset = bo; c->hier_size = bsz; c->num_points = np; }
+        else { c->byte_offset = bo; c->byte_size = bsz; c->num_points = np; }
+        c->type = type;
+        c->mask = mask;
+        if (type == 2) continue;
+        for (int k = 0; k < 8; k++) {
+            if (!(mask & (1 << k))) continue;
+            if (fill >= nrec) { rc = -1; break; }
+            const int ci = pool_add(p);
+            c = &p->n[list[i]];   /* the pool may have moved */
+            potree_node* ch = &p->n[ci];
+            child_box(c->box, k, ch->box);
+            ch->spacing = c->spacing / 2.0f;
+            ch->level = c->level + 1;
+            ch->parent = list[i];
+            c->child[k] = ci;
+            list[fill++] = ci;
+        }
+    }
+    for (int64_t i = 0; i < fill && rc == 0; i++)
+        if (p->n[list[i]].type == 2) rc = parse_chunk(p, list[i], h, hsize);
+    free(list);
+    return rc;
+}
+
+static void dfs_order(potree_pool* p, int i, uint64_t* counter) {
+    p->n[i].dfs = (*counter)++;
+    for (int k = 0; k < 8; k++) if (p->n[i].child[k] >= 0) dfs_order(p, p->n[i].child[k], counter);
+}
+
+static const potree_node* g_sort_nodes;
+static int cmp_offset(const void* a, const void* b) {
+    const potree_node* x = &g_sort_nodes[*(const int*)a];
+    const potree_node* y = &g_sort_nodes[*(const int*)b];
+    if (x->byte_offset != y->byte_offset) return x->byte_offset < y->byte_offset ? -1 : 1;
+    return x->dfs < y->dfs ? -1 : (x->dfs > y->dfs);
+}
+
+/* PotreeData.h:250-317 (traverse, then sort by byteOffset; ties keep the
+ * traversal order) and :368-380 (the batch header) */
+int64_t orc_potree_nodes(const pcr_potree_info* info, const uint8_t* h, uint64_t hsize, pcr_batch* nodes,
+                         double* boxes, uint64_t max_nodes) {
+    potree_pool p = {0, 0, 0};
+    const int root = pool_add(&p);
+    p.n[root].type = 2;
+    p.n[root].hier_offset = 0;
+    p.n[root].hier_size = (int64_t)info->first_chunk_size;
+    p.n[root].spacing = (float)info->spacing;
+    for (int a = 0; a < 3; a++) { p.n[root].box[a] = 0.0; p.n[root].box[3 + a] = info->max[a] - info->min[a]; }
+    int rc = parse_chunk(&p, root, h, hsize);
+    int64_t out = -1;
+    if (rc == 0 && p.count <= max_nodes && info->bytes_per_point > 0) {
+        uint64_t counter = 0;
+        dfs_order(&p, root, &counter);
+        int* idx = (int*)malloc(sizeof(int) * p.count);
+        for (uint64_t i = 0; i < p.count; i++) idx[i] = (int)i;
+        g_sort_nodes = p.n;
+        qsort(idx, p.count, sizeof(int), cmp_offset);
+        int* pos = (int*)malloc(sizeof(int) * p.count);
+        for (uint64_t i = 0; i < p.count; i++) pos[idx[i]] = (int)i;
+        out = (int64_t)p.count;
+        for (uint64_t i = 0; i < p.count && out >= 0; i++) {
+            const potree_node* n = &p.n[idx[i]];
+            if (n->byte_offset < 0 || n->byte_offset % info->bytes_per_point ||
+                (uint64_t)n->byte_offset / info->bytes_per_point + n->num_points > info->num_points) {
+                out = -1;
+                break;
+            }
+            pcr_batch* b = &nodes[i];
+            memset(b, 0, sizeof(*b));
+            for (int a = 0; a < 3; a++) { b->min[a] = (float)n->box[a]; b->max[a] = (float)n->box[3 + a]; }
+            b->num_points = n->num_points;
+            b->first_point = (uint32_t)(n->byte_offset / info->bytes_per_point);
+            b->level = (uint32_t)n->level;
+            b->parent = n->parent >= 0 ? pos[n->parent] : -1;
+            b->spacing = n->spacing;
+            for (int a = 0; a < 6; a++) boxes[6 * i + a] = n->box[a];
+        }
+        free(pos);
+        free(idx);
+    }
+    free(p.n);
+    return out;
+}
+
+/* PotreeData.h:439-511 with the clamped quantisation and bits 9..0 in the
+ * fine plane */
+void orc_potree_encode(const pcr_potree_info* info, const uint8_t* oct, const pcr_batch* nodes,
+                       const double* boxes, uint64_t nn, uint32_t* lo, uint32_t* med, uint32_t* hi, uint32_t* rgba) {
+    const uint64_t L = info->bytes_per_point;
+    for (uint64_t k = 0; k < nn; k++) {
+        const double* bx = boxes + 6 * k;
+        for (uint32_t j = 0; j < nodes[k].num_points; j++) {
+            const uint64_t i = (uint64_t)nodes[k].first_point + j;
+            const uint8_t* r = oct + i * L;
+            uint32_t q[3];
+            for (int a = 0; a < 3; a++) {
+                const double x = (double)(int32_t)rd_u32(r + 4 * a) * info->scale[a] + info->offset[a] - info->min[a];
+                orc_quantize30(x, bx[a], bx[3 + a] - bx[a], &q[a]);
+            }
+            lo[i]  = ((q[0] >> 20) & 1023u) | (((q[1] >> 20) & 1023u) << 10) | (((q[2] >> 20) & 1023u) << 20);
+            med[i] = ((q[0] >> 10) & 1023u) | (((q[1] >> 10) & 1023u) << 10) | (((q[2] >> 10) & 1023u) << 20);
+            hi[i]  = (q[0] & 1023u) | ((q[1] & 1023u) << 10) | ((q[2] & 1023u) << 20);
+            uint32_t c[3];
+            for (int m = 0; m < 3; m++) {
+                const uint32_t v = rd_u16(r + info->rgb_offset + 2 * m);
+                c[m] = v < 256 ? v : v / 256;
             }
             rgba[i] = c[0] | (c[1] << 8) | (c[2] << 16);
         }
@@ -568,6 +739,70 @@ void orc_render_nodes(const pcr_batch* nodes, uint32_t nn, const uint32_t* lo, c
         render_batch(&nodes[i], lo, med, hi, v, level, fb, 0);
     }
     if (!flags_out) free(flags);
+}
+
+/* compute_loop_nodes_hqs/depth.cs:225-360 and color.cs:600-760: the hqs
+ * passes over the nodes of the LOD cut, each at its node precision level
+ * (first threshold 80 px); the interactive selection/highlight colouring is
+ * not part of it. */
+void orc_hqs_nodes(const pcr_batch* nodes, uint32_t nn, const uint32_t* lo, const uint32_t* med,
+                   const uint32_t* hi, const uint32_t* rgba, const pcr_view* v, int culling, int rule,
+                   float lod_px, uint32_t* depth, uint32_t* acc) {
+    uint8_t* flags = (uint8_t*)malloc(nn ? nn : 1);
+    orc_lod_select(nodes, nn, v, culling, rule, lod_px, flags);
+    for (int pass = 0; pass < 2; pass++) {
+        for (uint32_t k = 0; k < nn; k++) {
+            if (!flags[k]) continue;
+            const int level = orc_precision_level(nodes[k].min, nodes[k].max, v, 80.0f);
+            for (uint32_t i = 0; i < nodes[k].num_points; i++) {
+                const uint32_t idx = nodes[k].first_point + i;
+                float p[3];
+                orc_decode(level, lo[idx], level <= 1 ? med[idx] : 0u, level == 0 ? hi[idx] : 0u, nodes[k].min,
+                           nodes[k].max, p);
+                if (pass == 0) hqs_depth_point(v, p, depth);
+                else hqs_color_point(v, p, rgba[idx], depth, acc);
+            }
+        }
+    }
+    free(flags);
+}
+
+/* compute_loop_nodes_hqs/resolve.cs:92-158: each pixel averages the window
+ * (2*window+1)^2 around it in float, neighbours weighted 100 (centre), 2
+ * (ox <= 1 && oy <= 1) or 1, and dropped when deeper than 1.01 x the
+ * window's smallest depth; neighbours outside the image are skipped (the
+ * reference reads past the buffer). No colour -> 0; alpha 0. */
+void orc_hqs_resolve_window(const uint32_t* depth, const uint32_t* acc, int w, int h, int window, uint32_t* out) {
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            float dmin = 1000000.0f;
+            for (int ox = -window; ox <= window; ox++)
+                for (int oy = -window; oy <= window; oy++) {
+                    const int px = x + ox, py = y + oy;
+                    if (px < 0 || py < 0 || px >= w || py >= h) continue;
+                    const float pd = bitsf(depth[(uint64_t)py * w + px]);
+                    if (pd >= 0.0f) dmin = pd < dmin ? pd : dmin;
+                }
+            float R = 0.0f, G = 0.0f, B = 0.0f, C = 0.0f;
+            for (int ox = -window; ox <= window; ox++)
+                for (int oy = -window; oy <= window; oy++) {
+                    const int px = x + ox, py = y + oy;
+                    if (px < 0 || py < 0 || px >= w || py >= h) continue;
+                    const uint64_t q = (uint64_t)py * w + px;
+                    const float pd = bitsf(depth[q]);
+                    float wt = (ox == 0 && oy == 0) ? 100.0f : (ox <= 1 && oy <= 1) ? 2.0f : 1.0f;
+                    if (pd > dmin * 1.01f) wt = 0.0f;
+                    R += (float)acc[4 * q + 0] * wt;
+                    G += (float)acc[4 * q + 1] * wt;
+                    B += (float)acc[4 * q + 2] * wt;
+                    C += (float)acc[4 * q + 3] * wt;
+                }
+            uint32_t color = 0u;
+            if (C != 0.0f)
+                color = (uint32_t)(R / C) | ((uint32_t)(G / C) << 8) | ((uint32_t)(B / C) << 16);
+            if (dmin < 0.0f) color = 0x0000FF00u;
+            out[(uint64_t)y * w + x] = color;
+        }
 }
 
 /* ------------------------------------------------------ multithreaded CPU */

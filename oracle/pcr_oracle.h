@@ -49,6 +49,23 @@ void orc_las_encode(const uint8_t* records, uint64_t n, const pcr_las_info* info
                     uint64_t sparse_offset, uint32_t file_index, pcr_batch* batches,
                     uint32_t* lo, uint32_t* med, uint32_t* hi, uint32_t* rgba);
 
+/* ---- LOD + hqs (compute_loop_nodes_hqs) --------------------------------- */
+void orc_hqs_nodes(const pcr_batch* nodes, uint32_t nn, const uint32_t* lo, const uint32_t* med,
+                   const uint32_t* hi, const uint32_t* rgba, const pcr_view* v, int culling, int rule,
+                   float lod_px, uint32_t* depth, uint32_t* acc);
+void orc_hqs_resolve_window(const uint32_t* depth, const uint32_t* acc, int w, int h, int window, uint32_t* out);
+
+/* ---- Potree 2.0 ingest (PotreeData.h:144-317, 439-511) ----------------- */
+/* Nodes of hierarchy.bin as batches in octree.bin order, with each node's
+ * double box (min xyz, max xyz) in boxes[6*i]. Returns the node count, or -1
+ * for a malformed hierarchy or more than max_nodes nodes. */
+int64_t orc_potree_nodes(const pcr_potree_info* info, const uint8_t* hier, uint64_t hsize,
+                         pcr_batch* nodes, double* boxes, uint64_t max_nodes);
+/* Encode every node's points (octree.bin records) into the planes. */
+void orc_potree_encode(const pcr_potree_info* info, const uint8_t* octree, const pcr_batch* nodes,
+                       const double* boxes, uint64_t nn, uint32_t* lo, uint32_t* med, uint32_t* hi,
+                       uint32_t* rgba);
+
 /* ---- per-batch stages --------------------------------------------------- */
 int   orc_frustum_visible(const float tf[16], const float mn[3], const float mx[3]);
 float orc_projected_px(const float mn[3], const float mx[3], float len,

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 #include "pcr_method.hpp"
 
@@ -39,7 +40,89 @@ static void perspective(double fovy_deg, double aspect, double n, double f, doub
     out[14] = -(2.0 * f * n) / (f - n);
 }
 
-int main() {
+// Renders `parts` of `cloud` with method `m` and directly (one pcr_render per
+// part, clear before the first); returns the number of drawn pixels, or -1
+// when the images differ.
+static long compare_parts(pcr_ctx* ctx, Renderer& r, BatchedMethod* m, pcr_cloud* cloud,
+                          const std::vector<CloudPart>& parts) {
+    View& v = r.views[0];
+    m->update(&r);
+    m->render(&r);
+    pcr_fb* fb = nullptr;
+    check(pcr_fb_create(ctx, v.width, v.height, PCR_FB_KEY64, &fb));
+    for (size_t k = 0; k < parts.size(); k++) {
+        pcr_view pv;
+        check(pcr_view_from_matrices(v.view, v.proj, parts[k].origin, v.width, v.height, &pv));
+        pcr_render_params p = m->params();
+        p.clear_before = k == 0;
+        p.batch_begin = parts[k].batch_begin;
+        p.batch_end = parts[k].batch_end;
+        check(pcr_render(ctx, cloud, &pv, &p, fb));
+    }
+    check(pcr_resolve(ctx, cloud, &m->params(), fb));
+    std::vector<uint32_t> img((size_t)v.width * v.height);
+    check(pcr_fb_readback_rgba(ctx, fb, img.data()));
+    pcr_fb_destroy(fb);
+    long drawn = 0;
+    for (uint32_t px : v.image) drawn += px != PCR_BG_LAS;
+    return img == v.image ? drawn : -1;
+}
+
+// method_main --las <file>...: LAS files through LasFiles + loop_las_prefetch.
+// method_main --potree <dir> <pcr_potree_info bytes>: a Potree 2.0 dataset
+// through PotreeCloud + loop_nodes. The camera looks at the first part's
+// origin from (150, -250, 200) m in front of it.
+static int run_files(int argc, char** argv) {
+    pcr_ctx* ctx = nullptr;
+    check(pcr_create(0, &ctx));
+    Renderer r;
+    r.ctx = ctx;
+    r.views.resize(1);
+    View& v = r.views[0];
+    v.width = 800;
+    v.height = 600;
+    int failures = 0;
+    auto camera = [&](const double o[3]) {
+        const double eye[3] = {o[0] + 150, o[1] - 250, o[2] + 200}, tgt[3] = {o[0] + 150, o[1] + 100, o[2]};
+        const double up[3] = {0, 0, 1};
+        look_at(eye, tgt, up, v.view);
+        perspective(60.0, 800.0 / 600.0, 1.0, 5000.0, v.proj);
+    };
+    if (std::string(argv[1]) == "--las") {
+        LasFiles las(ctx, std::vector<std::string>(argv + 2, argv + argc));
+        camera(las.parts[0].origin);
+        ComputeLoopLas2 m(ctx, las.cloud, las.parts);
+        const long drawn = compare_parts(ctx, r, &m, las.cloud, las.parts);
+        std::printf("%s las_files %zu %ld\n", drawn > 1000 ? "OK" : "FAIL", las.parts.size(), drawn);
+        failures += drawn <= 1000;
+    } else if (std::string(argv[1]) == "--potree" && argc == 4) {
+        const std::vector<char> raw = read_file(argv[3]);
+        if (raw.size() != sizeof(pcr_potree_info)) throw Error(PCR_E_INVALID, "bad pcr_potree_info file");
+        pcr_potree_info info;
+        std::memcpy(&info, raw.data(), sizeof(info));
+        PotreeCloud pc(ctx, info, argv[2]);
+        const double mid[3] = {pc.part.origin[0] + 362, pc.part.origin[1] + 512, pc.part.origin[2] + 300};
+        camera(mid);
+        ComputeLoopNodes m(ctx, pc.cloud, std::vector<CloudPart>{pc.part});
+        const long drawn = compare_parts(ctx, r, &m, pc.cloud, std::vector<CloudPart>{pc.part});
+        std::printf("%s potree %ld\n", drawn > 1000 ? "OK" : "FAIL", drawn);
+        failures += drawn <= 1000;
+    } else {
+        throw Error(PCR_E_INVALID, "usage: method_main [--las file... | --potree dir info]");
+    }
+    pcr_destroy(ctx);
+    return failures ? 1 : 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc > 1) {
+        try {
+            return run_files(argc, argv);
+        } catch (const Error& e) {
+            std::printf("FAIL pcr error %d: %s\n", e.code, e.what());
+            return 1;
+        }
+    }
     try {
         pcr_ctx* ctx = nullptr;
         check(pcr_create(0, &ctx));
@@ -59,6 +142,7 @@ int main() {
         Runtime::addMethod(new ComputeLoopLas2(ctx, cloud, origin));
         Runtime::addMethod(new ComputeLoopLasHqs(ctx, cloud, origin));
         Runtime::addMethod(new ComputeLoopNodes(ctx, tree, origin));
+        Runtime::addMethod(new ComputeLoopNodesHqs(ctx, tree, origin));
 
         Renderer r;
         r.ctx = ctx;
@@ -71,7 +155,7 @@ int main() {
         perspective(60.0, 800.0 / 450.0, 1.0, 5000.0, v.proj);
 
         int failures = 0;
-        for (const char* name : {"loop_las", "loop_las_prefetch", "loop_las_hqs", "loop_nodes"}) {
+        for (const char* name : {"loop_las", "loop_las_prefetch", "loop_las_hqs", "loop_nodes", "loop_nodes_hqs"}) {
             Runtime::setSelectedMethod(name);
             Method* m = Runtime::getSelectedMethod();
             if (!m || m->name != name) { std::printf("FAIL %s not registered\n", name); return 1; }
@@ -82,15 +166,18 @@ int main() {
             pcr_view pv;
             check(pcr_view_from_matrices(v.view, v.proj, origin, v.width, v.height, &pv));
             pcr_fb* fb = nullptr;
-            check(pcr_fb_create(ctx, v.width, v.height, bm->params().mode == PCR_MODE_HQS ? PCR_FB_HQS : PCR_FB_KEY64, &fb));
-            pcr_cloud* c = bm->params().mode == PCR_MODE_NODES ? tree : cloud;
+            const int md = bm->params().mode;
+            check(pcr_fb_create(ctx, v.width, v.height,
+                                md == PCR_MODE_HQS || md == PCR_MODE_NODES_HQS ? PCR_FB_HQS : PCR_FB_KEY64, &fb));
+            pcr_cloud* c = md == PCR_MODE_NODES || md == PCR_MODE_NODES_HQS ? tree : cloud;
             check(pcr_render(ctx, c, &pv, &bm->params(), fb));
             check(pcr_resolve(ctx, c, &bm->params(), fb));
             std::vector<uint32_t> img((size_t)v.width * v.height);
             check(pcr_fb_readback_rgba(ctx, fb, img.data()));
             pcr_fb_destroy(fb);
             size_t drawn = 0;
-            for (uint32_t p : v.image) drawn += p != PCR_BG_LAS;
+            const uint32_t bg = md == PCR_MODE_NODES_HQS ? 0u : PCR_BG_LAS;   // nodes_hqs resolves empty pixels to 0
+            for (uint32_t p : v.image) drawn += p != bg;
             const bool same = img == v.image;
             std::printf("%s %s %zu\n", same && drawn > 1000 ? "OK" : "FAIL", name, drawn);
             failures += !(same && drawn > 1000);
