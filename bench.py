@@ -265,6 +265,49 @@ def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
             "lod_px": 1.0, "mode": "nodes"}
 
 
+def ingest_leg(ctx, n_las, seed=5):
+    """LAS and Potree 2.0 ingest (host bytes -> device cloud, including the
+    host-to-device copy of the records), and LOD frames of the Potree cloud
+    with and without hqs."""
+    from compute_rasterizer_amd.datasets import make_las, make_potree
+    out = {}
+    specs = [(2, (1, 2)), (3, (1, 3)), (7, (1, 4)), (8, (1, 4))]
+    files = [make_las(n_las // 4 + k, point_format=f, version=v, seed=seed + k, extent=(2000.0, 2000.0, 200.0),
+                      offset=(500000.0 + 500.0 * k, 4000000.0, 100.0))[0] for k, (f, v) in enumerate(specs)]
+    warm, _ = ctx.from_las([make_las(1000, seed=seed)[0]])
+    warm.close()
+    t = time.perf_counter()
+    cloud, first = ctx.from_las(files)
+    las_s = time.perf_counter() - t
+    n = int(cloud.info().num_points)
+    out["las"] = {"files": len(files), "points": n, "bytes": sum(len(f) for f in files), "seconds": round(las_s, 3),
+                  "Mpoints_s": round(n / las_s / 1e6, 1), "batches": int(first[-1])}
+    cloud.close()
+    del files
+    meta, hier, octree, _ = make_potree(seed=seed, max_depth=5, points_per_node=(2000, 8000))
+    t = time.perf_counter()
+    pc = ctx.from_potree(meta, hier, octree)
+    pot_s = time.perf_counter() - t
+    info = pc.info()
+    out["potree"] = {"nodes": int(info.num_batches), "points": int(info.num_points), "bytes": len(octree) + len(hier),
+                     "seconds": round(pot_s, 3), "Mpoints_s": round(info.num_points / pot_s / 1e6, 1)}
+    pinfo = pcr.potree_info(meta)
+    o = tuple(pinfo.min)
+    v = pcr.look_at((o[0] + 512, o[1] - 600, o[2] + 700), (o[0] + 512, o[1] + 512, o[2] + 200), up=(0, 0, 1),
+                    near=1.0, far=10000.0, origin=o, width=1920, height=1080)
+    for name, mode, kind in (("nodes_ms", pcr.MODE_NODES, pcr.FB_KEY64), ("nodes_hqs_ms", pcr.MODE_NODES_HQS, pcr.FB_HQS)):
+        fb = ctx.framebuffer(1920, 1080, kind)
+        prm = pcr.params(mode, lod_px=1.0, time_stages=1, resolve_window=1)
+        for _ in range(20):
+            ctx.render(pc, v, prm, fb)
+            ctx.resolve(pc, prm, fb)
+        ft = ctx.frame_times(16)
+        out["potree"][name] = round(float(np.median(ft[:, 4])), 4)
+        fb.close()
+    pc.close()
+    return out
+
+
 def cpu_baseline(ctx, cloud, views, gpu_fb, prm, frames, n_points):
     """Oracle (multithreaded C) on the host cores for `frames` full frames;
     the GPU framebuffers of the same views are compared bit for bit."""
@@ -308,6 +351,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
     ap.add_argument("--serve-clouds", default="10e6,100e6,500e6,1e9")
+    ap.add_argument("--ingest-points", type=float, default=20e6, help="LAS ingest leg size (0 skips the leg)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 device copies of 1 GiB (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -433,6 +477,9 @@ def main():
         sizes = [int(float(x)) for x in args.serve_clouds.split(",")]
         out["serving"] = serving_leg(ctx, args.serve_seconds, args.serve_rate, args.serve_budget_gb, sizes)
         print(f"[bench] serving {out['serving']}", file=sys.stderr, flush=True)
+    if args.ingest_points > 0:
+        out["ingest"] = ingest_leg(ctx, int(args.ingest_points))
+        print(f"[bench] ingest {out['ingest']}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -23,7 +23,7 @@ def test_cpp_methods_render_through_runtime():
 def test_cpp_las_files_through_loop_las_prefetch(tmp_path):
     """LasFiles reads three LAS files into one cloud; loop_las_prefetch draws
     each file with its own origin, as a direct per-file render does."""
-    from tests.las_files import make_las
+    from compute_rasterizer_amd.datasets import make_las
     paths = []
     for k, (n, fmt, ver) in enumerate([(25_601, 2, (1, 2)), (13_333, 3, (1, 3)), (20_482, 7, (1, 4))]):
         data = make_las(n, point_format=fmt, version=ver, seed=10 + k, offset=(500000.0 + 40 * k, 4000000.0, 100.0))[0]
@@ -39,7 +39,7 @@ def test_cpp_potree_through_loop_nodes(tmp_path):
     """PotreeCloud reads a Potree 2.0 dataset; loop_nodes renders it as a
     direct render of the same cloud does."""
     import compute_rasterizer_amd as pcr
-    from tests.potree_files import make_potree
+    from compute_rasterizer_amd.datasets import make_potree
     meta, hier, octree, nodes = make_potree(seed=12, max_depth=5)
     (tmp_path / "hierarchy.bin").write_bytes(hier)
     (tmp_path / "octree.bin").write_bytes(octree)

@@ -11,7 +11,7 @@ import pytest
 import compute_rasterizer_amd as pcr
 from compute_rasterizer_amd import capi
 from oracle import orc
-from tests.las_files import make_las
+from compute_rasterizer_amd.datasets import make_las
 
 pytestmark = pytest.mark.gpu
 

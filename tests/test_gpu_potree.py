@@ -6,7 +6,7 @@ import pytest
 
 import compute_rasterizer_amd as pcr
 from oracle import orc
-from tests.potree_files import make_potree
+from compute_rasterizer_amd.datasets import make_potree
 
 pytestmark = pytest.mark.gpu
 

@@ -13,7 +13,7 @@ import pytest
 import compute_rasterizer_amd as pcr
 from compute_rasterizer_amd import capi
 from oracle import orc
-from tests.las_files import make_las
+from compute_rasterizer_amd.datasets import make_las
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "5points_14.las")
 

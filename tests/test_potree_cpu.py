@@ -9,7 +9,7 @@ import pytest
 
 import compute_rasterizer_amd as pcr
 from oracle import orc
-from tests.potree_files import make_potree
+from compute_rasterizer_amd.datasets import make_potree
 
 
 @pytest.mark.parametrize("seed,chunk_depth,max_depth", [(1, 2, 4), (2, 1, 3), (3, 3, 5), (4, 8, 3)])
