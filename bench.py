@@ -127,6 +127,71 @@ def multiview_leg(ctx, cloud, n_points, nviews, frames, tile_px):
             "bit_exact_vs_plain": bool(exact)}
 
 
+def stereo_pairs(n, width, height, ipd=0.065, seed=3, extent=2000.0, altitude=250.0):
+    """Two-eye views along a circular flythrough: each eye sits ipd/2 to the
+    side of the head position, both looking at the same target offset."""
+    rng = np.random.default_rng(seed)
+    c = extent / 2
+    pairs = []
+    for k in range(n):
+        a = 2 * math.pi * k / n + rng.uniform(-0.02, 0.02)
+        head = np.array([c + 650.0 * math.cos(a), c + 650.0 * math.sin(a), altitude + rng.uniform(-10, 10)])
+        ahead = a + math.pi / 2 + rng.uniform(-0.3, 0.3)
+        fwd = np.array([400 * math.cos(ahead), 400 * math.sin(ahead), 60.0 - head[2]])
+        right = np.array([fwd[1], -fwd[0], 0.0])
+        right *= (ipd / 2) / np.linalg.norm(right)
+        pairs.append([pcr.look_at(tuple(head + s * right), tuple(head + s * right + fwd), up=(0, 0, 1), fovy=90.0,
+                                  near=0.1, far=5000.0, width=width, height=height) for s in (-1.0, 1.0)])
+    return pairs
+
+
+def stereo_hqs_leg(ctx, cloud, n_points, frames, width, height):
+    """hqs for two eyes (the VR methods): per-eye renders vs one shared-decode
+    multi-view launch that reads each batch once for both eyes."""
+    pairs = stereo_pairs(frames, width, height)
+    prm = pcr.params(pcr.MODE_HQS, tile_px=2048)
+    plain_fbs = [ctx.framebuffer(width, height, pcr.FB_HQS) for _ in range(2)]
+    mv_fbs = [ctx.framebuffer(width, height, pcr.FB_HQS) for _ in range(2)]
+
+    def plain_frame(pair):
+        for v, fb in zip(pair, plain_fbs):
+            ctx.render(cloud, v, prm, fb)
+            ctx.resolve(cloud, prm, fb)
+
+    plain_frame(pairs[0])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for pair in pairs:
+        plain_frame(pair)
+    ctx.synchronize()
+    plain_s = (time.perf_counter() - t0) / frames
+    best, best_ms = pcr.autotune(ctx, cloud, pairs[0], mv_fbs, prm, iters=3)
+    mv = pcr.MultiView(ctx, cloud, pairs[0], mv_fbs, prm, tune=best)
+    mv.launch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for pair in pairs:
+        mv.set_views(pair)
+        mv.launch()
+    ctx.synchronize()
+    mv_s = (time.perf_counter() - t0) / frames
+    exact = True
+    for p, m in zip(plain_fbs, mv_fbs):   # both hold the last pair
+        pd, pa = p.hqs()
+        md, ma = m.hqs()
+        exact = exact and bool(np.array_equal(pd, md) and np.array_equal(pa, ma) and
+                               np.array_equal(p.rgba(), m.rgba()))
+    mv.close()
+    for fb in plain_fbs + mv_fbs:
+        fb.close()
+    return {"eyes": 2, "width": width, "height": height, "frames": frames,
+            "per_eye_ms_per_frame": round(plain_s * 1e3, 3), "shared_decode_ms_per_frame": round(mv_s * 1e3, 3),
+            "shared_decode_Gpoints_s": round(n_points * 2 / mv_s / 1e9, 1), "speedup": round(plain_s / mv_s, 3),
+            "tune": {"block_size": best.block_size, "points_per_thread": best.points_per_thread,
+                     "tile_px": best.tile_px, "median_ms": round(best_ms, 3)},
+            "bit_exact_vs_per_eye": exact}
+
+
 def layout_sweep(ctx, n_points, batch_sizes, nviews, frames):
     """Autotuned multi-view frame time per batch size (each a separate layout of
     the same 1e9-point scene; the batch AABBs, hence the decoded points, differ)."""
@@ -297,7 +362,7 @@ def ingest_leg(ctx, n_las, seed=5):
                     near=1.0, far=10000.0, origin=o, width=1920, height=1080)
     for name, mode, kind in (("nodes_ms", pcr.MODE_NODES, pcr.FB_KEY64), ("nodes_hqs_ms", pcr.MODE_NODES_HQS, pcr.FB_HQS)):
         fb = ctx.framebuffer(1920, 1080, kind)
-        prm = pcr.params(mode, lod_px=1.0, time_stages=1, resolve_window=1)
+        prm = pcr.params(mode, lod_px=1.0, time_stages=1, resolve_window=1, tile_px=2048)
         for _ in range(20):
             ctx.render(pc, v, prm, fb)
             ctx.resolve(pc, prm, fb)
@@ -345,6 +410,7 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="raster workgroup size (0 = default)")
     ap.add_argument("--mv-views", type=int, default=48, help="views of the batched multi-view leg (0 skips it)")
     ap.add_argument("--mv-frames", type=int, default=8)
+    ap.add_argument("--stereo-frames", type=int, default=16, help="frames of the two-eye hqs leg (0 skips it)")
     ap.add_argument("--mv-batch-sizes", default="5120,20480",
                     help="extra layouts for the multi-view autotune sweep ('' skips it)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="serving leg duration (0 skips it)")
@@ -463,6 +529,9 @@ def main():
     if args.mv_views > 0:
         out["multiview"] = multiview_leg(ctx, cloud, info.num_points, args.mv_views, args.mv_frames, args.tile_px)
         print(f"[bench] multiview {out['multiview']}", file=sys.stderr, flush=True)
+    if args.stereo_frames > 0:
+        out["stereo_hqs"] = stereo_hqs_leg(ctx, cloud, info.num_points, args.stereo_frames, W, H)
+        print(f"[bench] stereo hqs {out['stereo_hqs']}", file=sys.stderr, flush=True)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(ctx, cloud, views, fb, prm, args.cpu_frames, info.num_points)
     if args.mv_views > 0 and args.mv_batch_sizes:

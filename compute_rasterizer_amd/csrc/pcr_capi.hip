@@ -468,7 +468,10 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
                                  : launch_raster_batched(s, cloud, *view, 4, block ? block : 256, ctx->d_state, fb->d_keys);
             break;
         case PCR_MODE_HQS:
-        case PCR_MODE_NODES_HQS: rc = launch_hqs_batched(s, cloud, *view, ctx->d_state, fb); break;
+        case PCR_MODE_NODES_HQS:
+            rc = params->tile_px ? launch_hqs_tiled(s, cloud, *view, params->tile_px, block, ctx->d_state, fb)
+                                 : launch_hqs_batched(s, cloud, *view, ctx->d_state, fb);
+            break;
         case PCR_MODE_HQS16: rc = launch_hqs_points16(s, cloud, *view, fb); break;
     }
     if (rc) return rc;

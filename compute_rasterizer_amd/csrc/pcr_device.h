@@ -262,4 +262,63 @@ __device__ __forceinline__ void fb_min(unsigned long long* fb, uint32_t pixel, u
     if (key < old) atomicMin(fb + pixel, key);
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// hqs pass 1 (hqs/depth.cs:371-379): 32-bit atomicMin of the depth bits,
+// with the same early-out compare as fb_min.
+__device__ __forceinline__ void hqs_depth(uint32_t* __restrict__ depth, uint32_t pix, uint32_t d) {
+    if (d < depth[pix]) atomicMin(depth + pix, d);
+}
+// hqs pass 2 (hqs/color.cs:398-437): a point within 1.01x of the pixel's
+// minimum depth adds R, G, B and 1 to the pixel's four uint32 accumulators.
+__device__ __forceinline__ void hqs_accum(const uint32_t* __restrict__ depth, uint32_t* __restrict__ acc,
+                                          uint32_t pix, uint32_t d, uint32_t color) {
+    const float bd = __uint_as_float(depth[pix]);
+    if (!(__uint_as_float(d) <= bd * 1.01f)) return;
+    uint32_t* a = acc + 4ull * pix;
+    atomicAdd(a + 0, color & 255u);
+    atomicAdd(a + 1, (color >> 8) & 255u);
+    atomicAdd(a + 2, (color >> 16) & 255u);
+    atomicAdd(a + 3, 1u);
+}
+// hqs/resolve.cs:74-129: integer averages, background where count == 0.
+__device__ __forceinline__ uint32_t hqs_average(const u32x4 a, uint32_t bg) {
+    return a.w == 0 ? bg : ((a.x / a.w) | ((a.y / a.w) << 8) | ((a.z / a.w) << 16) | 0xFF000000u);
+}
+// compute_loop_nodes_hqs/resolve.cs:92-158: a float average over the
+// (2*window+1)^2 neighbourhood of pixel (x, y), weights 100 (centre),
+// 2 (ox <= 1 && oy <= 1) or 1, dropping neighbours deeper than 1.01 x the
+// window's smallest depth. Neighbours outside the image are skipped. Sums
+// run ox-major in the shader's order, so the float results are reproducible.
+__device__ inline uint32_t hqs_window_pixel(const uint32_t* __restrict__ depth, const u32x4* __restrict__ acc, int w,
+                                            int h, int x, int y, int window) {
+    float dmin = 1000000.0f;
+    for (int ox = -window; ox <= window; ox++)
+        for (int oy = -window; oy <= window; oy++) {
+            const int px = x + ox, py = y + oy;
+            if (px < 0 || py < 0 || px >= w || py >= h) continue;
+            const float pd = __uint_as_float(depth[(uint64_t)py * w + px]);
+            if (pd >= 0.0f) dmin = pd < dmin ? pd : dmin;
+        }
+    float R = 0.0f, G = 0.0f, B = 0.0f, C = 0.0f;
+    for (int ox = -window; ox <= window; ox++)
+        for (int oy = -window; oy <= window; oy++) {
+            const int px = x + ox, py = y + oy;
+            if (px < 0 || py < 0 || px >= w || py >= h) continue;
+            const uint64_t q = (uint64_t)py * w + px;
+            const float pd = __uint_as_float(depth[q]);
+            float wt = (ox == 0 && oy == 0) ? 100.0f : (ox <= 1 && oy <= 1) ? 2.0f : 1.0f;
+            if (pd > dmin * 1.01f) wt = 0.0f;
+            const u32x4 a = acc[q];
+            R += (float)a.x * wt;
+            G += (float)a.y * wt;
+            B += (float)a.z * wt;
+            C += (float)a.w * wt;
+        }
+    uint32_t color = 0u;
+    if (C != 0.0f) color = (uint32_t)(R / C) | ((uint32_t)(G / C) << 8) | ((uint32_t)(B / C) << 16);
+    if (dmin < 0.0f) color = 0x0000FF00u;
+    return color;
+}
+
 }  // namespace pcr

@@ -12,7 +12,8 @@ namespace pcr {
 // (a kernel rather than a memset so that the frame can be graph-captured).
 struct FrameState {
     uint32_t list_count;        // batches appended by the select stage
-    uint32_t queue_head;        // work items taken by the raster stage
+    uint32_t queue_head;        // work items taken by the raster stage (hqs: pass 1)
+    uint32_t queue_head2;       // hqs: work items taken by pass 2
     uint32_t level_count[5];
     uint32_t culled;
     unsigned long long points_drawn;
@@ -93,6 +94,8 @@ int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, in
                         FrameState* st, unsigned long long* fb);
 int launch_raster_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb);
 int launch_hqs_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb);
+int launch_hqs_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block, FrameState* st,
+                     pcr_fb* fb);
 int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pcr_fb* fb);
 int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, pcr_fb* fb);
 int launch_selftest_divide(hipStream_t s, const float* nx, const float* ny, const float* den, uint64_t n,
@@ -101,11 +104,14 @@ int launch_selftest_divide(hipStream_t s, const float* nx, const float* ny, cons
 // multi-view frame (pcr_multiview.hip), shared with the server
 struct MVState {
     uint32_t list_count;   // batches drawn by at least one view
-    uint32_t queue_head;   // work items taken by the persistent raster
+    uint32_t queue_head;   // work items taken by the persistent raster (hqs: pass 1)
+    uint32_t queue_head2;  // hqs: work items taken by pass 2
 };
 struct MVBuffers {
     pcr_view* d_views = nullptr;
     unsigned long long** d_fbs = nullptr;
+    uint32_t** d_depths = nullptr;   // hqs framebuffers: depth and accumulators
+    uint32_t** d_accs = nullptr;
     uint32_t** d_out = nullptr;
     float* d_lod = nullptr;
     uint8_t* d_levels = nullptr;
@@ -121,6 +127,8 @@ int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nv
                    const pcr_render_params& prm, bool lod, const pcr_tune& t, int cu_count);
 int launch_mv_raster_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
                            int cu_count);
+int launch_mv_hqs_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
+                        int cu_count);
 
 // layout (pcr_layout.hip)
 int generate_points(pcr_ctx* ctx, const pcr_gen_cfg* cfg, float* xyz, uint32_t* rgba);

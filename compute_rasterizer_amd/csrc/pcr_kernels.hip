@@ -14,7 +14,6 @@
 
 namespace pcr {
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ u32x4 ld_stream(const u32x4* p) { return __builtin_nontemporal_load(p); }
 
@@ -23,6 +22,14 @@ __device__ __forceinline__ u32x4 ld_stream(const u32x4* p) { return __builtin_no
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 __device__ __forceinline__ void lds_min(unsigned long long* p, unsigned long long v) {
     __hip_atomic_fetch_min((lds_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// ds_min_u32 and ds_add_u64, for the hqs tiles.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+__device__ __forceinline__ void lds_min32(uint32_t* p, uint32_t v) {
+    __hip_atomic_fetch_min((lds_u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_add((lds_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
@@ -39,7 +46,7 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 // The frame's counters reset with the clear (one launch fewer per frame):
 // thread 0 of workgroup 0 zeroes them when `st` is given.
 __device__ __forceinline__ void frame_reset(FrameState* st) {
-    st->list_count = 0; st->queue_head = 0; st->culled = 0;
+    st->list_count = 0; st->queue_head = 0; st->queue_head2 = 0; st->culled = 0;
     for (int i = 0; i < 5; i++) st->level_count[i] = 0;
     st->points_drawn = 0; st->plane_bytes = 0;
 }
@@ -535,6 +542,324 @@ int launch_mv_raster_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B
     return PCR_OK;
 }
 
+// ------------------------------------------------------------- tiled hqs
+// The LDS tile of the raster, for the two hqs passes. A batch whose
+// projected rectangle covers at most TILE_PIX pixels
+//   pass 1: min-reduces its depths in an LDS copy of the rectangle
+//           (ds_min_u32), then flushes each touched pixel with the compare +
+//           global atomicMin of hqs_depth;
+//   pass 2: loads the rectangle's final depths into LDS, accumulates its
+//           visible points there as two packed 64-bit sums per pixel,
+//           R | G << 32 and B | count << 32 (ds_add_u64), then adds each
+//           touched pixel's four sums to the global uint32 accumulators.
+// A batch of at most kHqsTileMaxPoints points cannot carry out of a 32-bit
+// half (255 x 16,843,009 < 2^32), and uint32 addition is associative mod
+// 2^32, so depth and accumulators are bit-identical to the direct passes.
+// Pass 2 is where the direct path spends its time: at a distance most points
+// of a pixel lie within 1.01x of its depth, and every one of them made four
+// contended global atomics.
+constexpr uint32_t kHqsTileMaxPoints = 16843009u;
+
+template <int PASS, int LC, bool TILE>
+__device__ __forceinline__ void hqs_tile_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi,
+                                               uint32_t color, uint32_t local, uint32_t np, const Proj& pj,
+                                               uint32_t* __restrict__ depth, uint32_t* __restrict__ acc,
+                                               uint32_t* tdepth, unsigned long long* tacc, int x0, int y0, int tw,
+                                               int th) {
+    if (local >= np) return;
+    float p[3];
+    decode<LC>(d, lo, med, hi, p);
+    int ix, iy;
+    uint32_t dep;
+    if (!project_xy(pj, p[0], p[1], p[2], ix, iy, dep)) return;
+    const uint32_t ux = (uint32_t)(ix - x0), uy = (uint32_t)(iy - y0);
+    if (TILE && ux < (uint32_t)tw && uy < (uint32_t)th) {
+        const uint32_t t = uy * (uint32_t)tw + ux;
+        if (PASS == 0) {
+            lds_min32(tdepth + t, dep);
+        } else {
+            if (!(__uint_as_float(dep) <= __uint_as_float(tdepth[t]) * 1.01f)) return;
+            lds_add64(tacc + 2 * t, (unsigned long long)(color & 255u) | ((unsigned long long)((color >> 8) & 255u) << 32));
+            lds_add64(tacc + 2 * t + 1, (unsigned long long)((color >> 16) & 255u) | (1ull << 32));
+        }
+    } else {
+        const uint32_t pix = (uint32_t)ix + (uint32_t)iy * (uint32_t)pj.w;
+        if (PASS == 0) hqs_depth(depth, pix, dep);
+        else hqs_accum(depth, acc, pix, dep, color);
+    }
+}
+
+template <int BLOCK, int PASS, int LC, bool TILE, bool NT>
+__device__ __forceinline__ void hqs_tile_loop(const Decode& d, const uint32_t* __restrict__ lo,
+                                              const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                              const uint32_t* __restrict__ rgba, uint32_t first, uint32_t np,
+                                              const Proj& pj, uint32_t* __restrict__ depth,
+                                              uint32_t* __restrict__ acc, uint32_t* tdepth, unsigned long long* tacc,
+                                              int x0, int y0, int tw, int th) {
+    const uint32_t lead = first & 3u;   // see tile_loop()
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
+    const u32x4* c4 = reinterpret_cast<const u32x4*>(rgba + (first - lead));
+    const uint32_t nq = (np + lead + 3u) >> 2;
+    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
+        const u32x4 a = ld_plane<NT>(lo4 + q);
+        const u32x4 m = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 col = PASS == 1 ? ld_plane<NT>(c4 + q) : u32x4{0, 0, 0, 0};
+        const uint32_t base = q * 4u - lead;
+        hqs_tile_point<PASS, LC, TILE>(d, a.x, m.x, c.x, col.x, base + 0, np, pj, depth, acc, tdepth, tacc, x0, y0, tw, th);
+        hqs_tile_point<PASS, LC, TILE>(d, a.y, m.y, c.y, col.y, base + 1, np, pj, depth, acc, tdepth, tacc, x0, y0, tw, th);
+        hqs_tile_point<PASS, LC, TILE>(d, a.z, m.z, c.z, col.z, base + 2, np, pj, depth, acc, tdepth, tacc, x0, y0, tw, th);
+        hqs_tile_point<PASS, LC, TILE>(d, a.w, m.w, c.w, col.w, base + 3, np, pj, depth, acc, tdepth, tacc, x0, y0, tw, th);
+    }
+}
+
+template <int BLOCK, int PASS, bool TILE, bool NT>
+__device__ __forceinline__ void hqs_tile_dispatch(int level, const Decode& d, const uint32_t* __restrict__ lo,
+                                                  const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                                  const uint32_t* __restrict__ rgba, const pcr_batch& h,
+                                                  const Proj& pj, uint32_t* __restrict__ depth,
+                                                  uint32_t* __restrict__ acc, uint32_t* tdepth,
+                                                  unsigned long long* tacc, int x0, int y0, int tw, int th) {
+    if (level == 0)
+        hqs_tile_loop<BLOCK, PASS, 0, TILE, NT>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth, acc,
+                                                tdepth, tacc, x0, y0, tw, th);
+    else if (level == 1)
+        hqs_tile_loop<BLOCK, PASS, 1, TILE, NT>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth, acc,
+                                                tdepth, tacc, x0, y0, tw, th);
+    else
+        hqs_tile_loop<BLOCK, PASS, 2, TILE, NT>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth, acc,
+                                                tdepth, tacc, x0, y0, tw, th);
+}
+
+// Pass 2 keeps 20 B per tile pixel; its tile holds 8 pixels fewer than
+// TILE_PIX so that with the kernels' other shared words four 256-thread
+// workgroups still fit a CU's 160 KB of LDS at TILE_PIX = 2048.
+template <int PASS, int TILE_PIX>
+struct HqsTile {
+    static constexpr int kCap = PASS == 1 ? TILE_PIX - 8 : TILE_PIX;
+    uint32_t depth[kCap];
+    unsigned long long acc[PASS == 1 ? 2 * kCap : 1];
+};
+
+// One batch of one hqs pass; `bi` and `level` are workgroup-uniform, and so
+// is `use` (every thread derives it from the same header and view).
+template <int BLOCK, int TILE_PIX, int PASS, bool NT>
+__device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
+                                                const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
+                                                const uint32_t* __restrict__ hi, const uint32_t* __restrict__ rgba,
+                                                const pcr_view& v, uint32_t* __restrict__ depth,
+                                                uint32_t* __restrict__ acc, uint32_t* tdepth,
+                                                unsigned long long* tacc) {
+    const pcr_batch h = b[bi];
+    const Decode d = make_decode(level, h.min, h.max);
+    const Proj pj = make_proj(v.transform, v.width, v.height);
+    int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
+    bool use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1);
+    const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
+    constexpr int cap = HqsTile<PASS, TILE_PIX>::kCap;
+    use = use && tw <= cap && th <= cap && tw * th <= cap && h.num_points <= kHqsTileMaxPoints;
+    if (!use) {
+        hqs_tile_dispatch<BLOCK, PASS, false, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, nullptr, nullptr, 0,
+                                                  0, 0, 0);
+        return;
+    }
+    const int n = tw * th;
+    // row = i / tw as a multiply-high (see tiled_batch)
+    const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        if (PASS == 0) {
+            tdepth[i] = PCR_DEPTH_CLEAR;
+        } else {
+            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
+            tdepth[i] = depth[(uint32_t)x0 + ux + ((uint32_t)y0 + uy) * (uint32_t)v.width];
+            tacc[2 * i] = 0ull;
+            tacc[2 * i + 1] = 0ull;
+        }
+    }
+    __syncthreads();
+    hqs_tile_dispatch<BLOCK, PASS, true, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, tdepth, tacc, x0, y0, tw,
+                                             th);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
+        const uint32_t pix = (uint32_t)x0 + ux + ((uint32_t)y0 + uy) * (uint32_t)v.width;
+        if (PASS == 0) {
+            const uint32_t t = tdepth[i];
+            if (t != PCR_DEPTH_CLEAR) hqs_depth(depth, pix, t);
+        } else {
+            const unsigned long long rg = tacc[2 * i], bc = tacc[2 * i + 1];
+            if ((bc >> 32) != 0ull) {
+                uint32_t* a = acc + 4ull * pix;
+                if ((uint32_t)rg) atomicAdd(a + 0, (uint32_t)rg);
+                if ((uint32_t)(rg >> 32)) atomicAdd(a + 1, (uint32_t)(rg >> 32));
+                if ((uint32_t)bc) atomicAdd(a + 2, (uint32_t)bc);
+                atomicAdd(a + 3, (uint32_t)(bc >> 32));
+            }
+        }
+    }
+}
+
+// Persistent workgroups over the select stage's list, in the spread order of
+// k_raster_tiled; pass 1 takes items from queue_head, pass 2 (launched after
+// pass 1 has finished) from queue_head2.
+template <int BLOCK, int TILE_PIX, int PASS>
+__global__ __launch_bounds__(BLOCK) void k_hqs_tiled(const pcr_batch* __restrict__ b,
+                                                     const uint8_t* __restrict__ levels,
+                                                     const uint32_t* __restrict__ list, FrameState* __restrict__ st,
+                                                     const uint32_t* __restrict__ lo,
+                                                     const uint32_t* __restrict__ med,
+                                                     const uint32_t* __restrict__ hi,
+                                                     const uint32_t* __restrict__ rgba, pcr_view v,
+                                                     uint32_t* __restrict__ depth, uint32_t* __restrict__ acc) {
+    __shared__ HqsTile<PASS, TILE_PIX> tile;
+    __shared__ uint32_t s_item;
+    uint32_t* head = PASS == 0 ? &st->queue_head : &st->queue_head2;
+    const uint32_t count = st->list_count;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= count) break;
+        const uint32_t slot = (uint32_t)(((uint64_t)item * 2654435761ull) % count);
+        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[slot]);
+        const int level = __builtin_amdgcn_readfirstlane(levels[bi]);
+        hqs_tiled_batch<BLOCK, TILE_PIX, PASS, true>(bi, level, b, lo, med, hi, rgba, v, depth, acc, tile.depth,
+                                                      tile.acc);
+        __syncthreads();   // s_item and the tile are reused by the next item
+    }
+}
+
+template <int BLOCK, int TILE_PIX, int PASS>
+static void hqs_tiled_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb) {
+    static const int occ = [] {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_hqs_tiled<BLOCK, TILE_PIX, PASS>, BLOCK, 0) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            o = 0;
+        }
+        return o > 0 ? o : 4;
+    }();
+    uint64_t grid = (uint64_t)c->ctx->cu_count * (uint64_t)occ;
+    if (grid > c->num_batches) grid = c->num_batches;
+    k_hqs_tiled<BLOCK, TILE_PIX, PASS><<<(int)grid, BLOCK, 0, s>>>(c->d_batches, c->d_levels, c->d_list, st, c->d_lo,
+                                                                   c->d_med, c->d_hi, c->d_rgba, v, fb->d_depth,
+                                                                   fb->d_acc);
+}
+
+template <int BLOCK, int TILE_PIX>
+static void hqs_tiled_passes(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb) {
+    hqs_tiled_t<BLOCK, TILE_PIX, 0>(s, c, v, st, fb);
+    hqs_tiled_t<BLOCK, TILE_PIX, 1>(s, c, v, st, fb);
+}
+
+int launch_hqs_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block, FrameState* st,
+                     pcr_fb* fb) {
+    if (c->num_batches == 0) return PCR_OK;
+    if (block == 128) {
+        if (tile_px == 1024) hqs_tiled_passes<128, 1024>(s, c, v, st, fb);
+        else if (tile_px == 4096) hqs_tiled_passes<128, 4096>(s, c, v, st, fb);
+        else hqs_tiled_passes<128, 2048>(s, c, v, st, fb);
+    } else {
+        if (tile_px == 1024) hqs_tiled_passes<256, 1024>(s, c, v, st, fb);
+        else if (tile_px == 4096) hqs_tiled_passes<256, 4096>(s, c, v, st, fb);
+        else hqs_tiled_passes<256, 2048>(s, c, v, st, fb);
+    }
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
+// Multi-view tiled hqs: one pass of k_hqs_tiled over every view that draws
+// the batch, as k_mv_raster_tiled does for keys. Pass 1 takes items from
+// queue_head, pass 2 from queue_head2.
+template <int BLOCK, int TILE_PIX, int PASS>
+__global__ __launch_bounds__(BLOCK) void k_mv_hqs_tiled(const pcr_batch* __restrict__ b, uint32_t nb,
+                                                        const uint8_t* __restrict__ levels,
+                                                        const uint32_t* __restrict__ list, MVState* __restrict__ st,
+                                                        const uint32_t* __restrict__ lo,
+                                                        const uint32_t* __restrict__ med,
+                                                        const uint32_t* __restrict__ hi,
+                                                        const uint32_t* __restrict__ rgba,
+                                                        const pcr_view* __restrict__ views, int nviews,
+                                                        uint32_t* const* __restrict__ depths,
+                                                        uint32_t* const* __restrict__ accs) {
+    __shared__ HqsTile<PASS, TILE_PIX> tile;
+    __shared__ uint32_t s_item, s_nv;
+    __shared__ uint8_t s_vid[64], s_lv[64];
+    uint32_t* head = PASS == 0 ? &st->queue_head : &st->queue_head2;
+    const uint32_t count = st->list_count;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= count) break;
+        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[item]);
+        if (threadIdx.x < 64) {   // compact the views drawing this batch (wave 0)
+            const int v = (int)threadIdx.x;
+            const int lv = v < nviews ? levels[(uint64_t)v * nb + bi] : kCulled;
+            const unsigned long long m = __ballot(lv != kCulled);
+            if (lv != kCulled) {
+                const uint32_t pos = (uint32_t)__popcll(m & ((1ull << v) - 1ull));
+                s_vid[pos] = (uint8_t)v;
+                s_lv[pos] = (uint8_t)lv;
+            }
+            if (v == 0) s_nv = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        const uint32_t nv = __builtin_amdgcn_readfirstlane(s_nv);
+        for (uint32_t k = 0; k < nv; k++) {
+            const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
+            const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
+            hqs_tiled_batch<BLOCK, TILE_PIX, PASS, false>(bi, lv, b, lo, med, hi, rgba, views[vi], depths[vi],
+                                                           accs[vi], tile.depth, tile.acc);
+            __syncthreads();   // the tile is reused by the next view
+        }
+    }
+}
+
+template <int BLOCK, int TILE_PIX, int PASS>
+static void mv_hqs_tiled_t(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int cu_count) {
+    static const int occ = [] {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_mv_hqs_tiled<BLOCK, TILE_PIX, PASS>, BLOCK, 0) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            o = 0;
+        }
+        return o > 0 ? o : 4;
+    }();
+    uint64_t grid = (uint64_t)cu_count * (uint64_t)occ;
+    if (grid > c->num_batches) grid = c->num_batches;
+    if (grid == 0) grid = 1;
+    k_mv_hqs_tiled<BLOCK, TILE_PIX, PASS><<<(int)grid, BLOCK, 0, s>>>(c->d_batches, c->num_batches, B.d_levels,
+                                                                      B.d_list, B.d_state, c->d_lo, c->d_med, c->d_hi,
+                                                                      c->d_rgba, B.d_views, nviews, B.d_depths,
+                                                                      B.d_accs);
+}
+
+template <int BLOCK, int TILE_PIX>
+static void mv_hqs_tiled_passes(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int cu_count) {
+    mv_hqs_tiled_t<BLOCK, TILE_PIX, 0>(s, c, B, nviews, cu_count);
+    mv_hqs_tiled_t<BLOCK, TILE_PIX, 1>(s, c, B, nviews, cu_count);
+}
+
+int launch_mv_hqs_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
+                        int cu_count) {
+    if (block == 128) {
+        if (tile_px == 1024) mv_hqs_tiled_passes<128, 1024>(s, c, B, nviews, cu_count);
+        else if (tile_px == 4096) mv_hqs_tiled_passes<128, 4096>(s, c, B, nviews, cu_count);
+        else mv_hqs_tiled_passes<128, 2048>(s, c, B, nviews, cu_count);
+    } else {
+        if (tile_px == 1024) mv_hqs_tiled_passes<256, 1024>(s, c, B, nviews, cu_count);
+        else if (tile_px == 4096) mv_hqs_tiled_passes<256, 4096>(s, c, B, nviews, cu_count);
+        else mv_hqs_tiled_passes<256, 2048>(s, c, B, nviews, cu_count);
+    }
+    PCR_HIP(hipGetLastError());
+    return PCR_OK;
+}
+
 int launch_raster_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int vec, int block,
                           FrameState* st, unsigned long long* fb) {
     (void)st;
@@ -582,23 +907,7 @@ int launch_raster_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v,
 }
 
 // -------------------------------------------------------------------- hqs
-// Pass 1 (depth.cs:371-379): 32-bit atomicMin of the depth bits.
-// Pass 2 (color.cs:398-437): points within 1.01x of the pixel's minimum depth
-// add R, G, B and 1 to four uint32 accumulators.
-__device__ __forceinline__ void hqs_depth(uint32_t* __restrict__ depth, uint32_t pix, uint32_t d) {
-    if (d < depth[pix]) atomicMin(depth + pix, d);
-}
-__device__ __forceinline__ void hqs_accum(const uint32_t* __restrict__ depth, uint32_t* __restrict__ acc,
-                                          uint32_t pix, uint32_t d, uint32_t color) {
-    const float bd = __uint_as_float(depth[pix]);
-    if (!(__uint_as_float(d) <= bd * 1.01f)) return;
-    uint32_t* a = acc + 4ull * pix;
-    atomicAdd(a + 0, color & 255u);
-    atomicAdd(a + 1, (color >> 8) & 255u);
-    atomicAdd(a + 2, (color >> 16) & 255u);
-    atomicAdd(a + 3, 1u);
-}
-
+// The per-pixel operations (hqs_depth, hqs_accum) are in pcr_device.h.
 template <int PASS, int LC>
 __device__ __forceinline__ void hqs_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi, uint32_t color,
                                           uint32_t local, uint32_t np, const pcr_view& v,
@@ -716,54 +1025,20 @@ __global__ void k_resolve_color(const unsigned long long* __restrict__ fb, uint6
         out[i] = k == PCR_FB_CLEAR ? bg : (uint32_t)k;
     }
 }
-// hqs/resolve.cs:74-129: integer averages, background where count == 0.
+// hqs/resolve.cs:74-129 (hqs_average).
 __global__ void k_resolve_hqs(const u32x4* __restrict__ acc, uint64_t npix, uint32_t bg, uint32_t* __restrict__ out) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
-        const u32x4 a = acc[i];
-        out[i] = a.w == 0 ? bg : ((a.x / a.w) | ((a.y / a.w) << 8) | ((a.z / a.w) << 16) | 0xFF000000u);
-    }
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride)
+        out[i] = hqs_average(acc[i], bg);
 }
 
-// compute_loop_nodes_hqs/resolve.cs:92-158: a float average over the
-// (2*window+1)^2 neighbourhood, weights 100 (centre), 2 (ox <= 1 && oy <= 1)
-// or 1, dropping neighbours deeper than 1.01 x the window's smallest depth.
-// Neighbours outside the image are skipped. Sums run ox-major in the
-// shader's order, so the float results are reproducible.
+// compute_loop_nodes_hqs/resolve.cs:92-158 (hqs_window_pixel).
 __global__ void k_resolve_hqs_window(const uint32_t* __restrict__ depth, const u32x4* __restrict__ acc, int w, int h,
                                      int window, uint32_t* __restrict__ out) {
     const uint64_t npix = (uint64_t)w * h;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
-        const int x = (int)(i % (uint64_t)w), y = (int)(i / (uint64_t)w);
-        float dmin = 1000000.0f;
-        for (int ox = -window; ox <= window; ox++)
-            for (int oy = -window; oy <= window; oy++) {
-                const int px = x + ox, py = y + oy;
-                if (px < 0 || py < 0 || px >= w || py >= h) continue;
-                const float pd = __uint_as_float(depth[(uint64_t)py * w + px]);
-                if (pd >= 0.0f) dmin = pd < dmin ? pd : dmin;
-            }
-        float R = 0.0f, G = 0.0f, B = 0.0f, C = 0.0f;
-        for (int ox = -window; ox <= window; ox++)
-            for (int oy = -window; oy <= window; oy++) {
-                const int px = x + ox, py = y + oy;
-                if (px < 0 || py < 0 || px >= w || py >= h) continue;
-                const uint64_t q = (uint64_t)py * w + px;
-                const float pd = __uint_as_float(depth[q]);
-                float wt = (ox == 0 && oy == 0) ? 100.0f : (ox <= 1 && oy <= 1) ? 2.0f : 1.0f;
-                if (pd > dmin * 1.01f) wt = 0.0f;
-                const u32x4 a = acc[q];
-                R += (float)a.x * wt;
-                G += (float)a.y * wt;
-                B += (float)a.z * wt;
-                C += (float)a.w * wt;
-            }
-        uint32_t color = 0u;
-        if (C != 0.0f) color = (uint32_t)(R / C) | ((uint32_t)(G / C) << 8) | ((uint32_t)(B / C) << 16);
-        if (dmin < 0.0f) color = 0x0000FF00u;
-        out[i] = color;
-    }
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride)
+        out[i] = hqs_window_pixel(depth, acc, w, h, (int)(i % (uint64_t)w), (int)(i / (uint64_t)w), window);
 }
 
 int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, pcr_fb* fb) {

@@ -13,6 +13,12 @@
 //                words at its own level, so each framebuffer is bit-identical
 //                to a single-view render of that view.
 //   k_mv_clear / k_mv_resolve cover all views in one launch each.
+//   hqs modes (HQS, NODES_HQS): k_mv_hqs<PASS> (or, with a tile size,
+//                k_mv_hqs_tiled in pcr_kernels.hip) runs the depth pass and
+//                then the accumulate pass over the same list, reading each
+//                batch's planes (and, in pass 2, its colours) once for all
+//                views, as the VR method does for its two eyes;
+//                k_mv_hqs_clear and k_mv_resolve_hqs cover all views.
 // The frame (clear, reset, select, raster, resolve) is captured once in a
 // hipGraph; views change through a device buffer copied before each launch.
 #include <algorithm>
@@ -22,11 +28,10 @@
 
 namespace pcr {
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kMaxViews = 64;
 
 __global__ void k_mv_reset(MVState* st) {
-    if (threadIdx.x == 0) { st->list_count = 0; st->queue_head = 0; }
+    if (threadIdx.x == 0) { st->list_count = 0; st->queue_head = 0; st->queue_head2 = 0; }
 }
 
 __global__ void k_mv_clear(unsigned long long* const* __restrict__ fbs, const pcr_view* __restrict__ views) {
@@ -198,9 +203,139 @@ __global__ void k_mv_resolve(const unsigned long long* const* __restrict__ fbs, 
     }
 }
 
+// ------------------------------------------------------------ hqs views
+__global__ void k_mv_hqs_clear(uint32_t* const* __restrict__ depths, uint32_t* const* __restrict__ accs,
+                               const pcr_view* __restrict__ views) {
+    const uint32_t v = blockIdx.y;
+    const uint64_t npix = (uint64_t)views[v].width * views[v].height;
+    uint32_t* dp = depths[v];
+    u32x4* ac = reinterpret_cast<u32x4*>(accs[v]);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (uint64_t)gridDim.x * blockDim.x) {
+        dp[i] = PCR_DEPTH_CLEAR;
+        ac[i] = u32x4{0, 0, 0, 0};
+    }
+}
+
+template <int PASS, int LC>
+__device__ __forceinline__ void mv_hqs_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi, uint32_t color,
+                                             uint32_t local, uint32_t cnt, const pcr_view& v,
+                                             uint32_t* __restrict__ depth, uint32_t* __restrict__ acc) {
+    if (local >= cnt) return;
+    float p[3];
+    decode<LC>(d, lo, med, hi, p);
+    uint32_t pix, dep;
+    if (!project(v.transform, p[0], p[1], p[2], v.width, v.height, pix, dep)) return;
+    if (PASS == 0) hqs_depth(depth, pix, dep);
+    else hqs_accum(depth, acc, pix, dep, color);
+}
+
+template <int PASS, int LC>
+__device__ __forceinline__ void mv_hqs_points4(const Decode& d, const u32x4& a, const u32x4& m, const u32x4& c,
+                                               const u32x4& col, uint32_t base, uint32_t cnt, const pcr_view& v,
+                                               uint32_t* __restrict__ depth, uint32_t* __restrict__ acc) {
+    mv_hqs_point<PASS, LC>(d, a.x, m.x, c.x, col.x, base + 0, cnt, v, depth, acc);
+    mv_hqs_point<PASS, LC>(d, a.y, m.y, c.y, col.y, base + 1, cnt, v, depth, acc);
+    mv_hqs_point<PASS, LC>(d, a.z, m.z, c.z, col.z, base + 2, cnt, v, depth, acc);
+    mv_hqs_point<PASS, LC>(d, a.w, m.w, c.w, col.w, base + 3, cnt, v, depth, acc);
+}
+
+// One hqs pass over every view: PASS 0 takes items from queue_head and
+// min-reduces depth, PASS 1 (launched after PASS 0 has finished, so every
+// view's depth is final) takes items from queue_head2 and accumulates the
+// points within 1.01x of that depth. Same (batch, chunk) items, spread order
+// and per-view levels as k_mv_raster.
+template <int BLOCK, int PASS>
+__global__ __launch_bounds__(BLOCK) void k_mv_hqs(const pcr_batch* __restrict__ b, uint32_t nb,
+                                                  const uint8_t* __restrict__ levels, const uint8_t* __restrict__ best,
+                                                  const uint32_t* __restrict__ list, MVState* __restrict__ st,
+                                                  const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
+                                                  const uint32_t* __restrict__ hi, const uint32_t* __restrict__ rgba,
+                                                  const pcr_view* __restrict__ views, int nviews,
+                                                  uint32_t* const* __restrict__ depths,
+                                                  uint32_t* const* __restrict__ accs, uint32_t chunk_pts,
+                                                  uint32_t chunks_per_batch) {
+    __shared__ uint32_t s_item;
+    __shared__ uint8_t s_vid[kMaxViews];
+    __shared__ uint8_t s_lv[kMaxViews];
+    __shared__ uint32_t s_nv;
+    uint32_t* head = PASS == 0 ? &st->queue_head : &st->queue_head2;
+    const uint32_t total = st->list_count * chunks_per_batch;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= total) break;
+        const uint32_t sp = (uint32_t)(((uint64_t)item * 2654435761ull) % total);
+        const uint32_t bi = __builtin_amdgcn_readfirstlane(list[sp / chunks_per_batch]);
+        const uint32_t chunk = sp % chunks_per_batch;
+        if (threadIdx.x < 64) {
+            const int v = (int)threadIdx.x;
+            const int lv = v < nviews ? levels[(uint64_t)v * nb + bi] : kCulled;
+            const unsigned long long m = __ballot(lv != kCulled);
+            if (lv != kCulled) {
+                const uint32_t pos = (uint32_t)__popcll(m & ((1ull << v) - 1ull));
+                s_vid[pos] = (uint8_t)v;
+                s_lv[pos] = (uint8_t)lv;
+            }
+            if (v == 0) s_nv = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        const pcr_batch h = b[bi];
+        const uint32_t start = chunk * chunk_pts;
+        const uint32_t nv = __builtin_amdgcn_readfirstlane(s_nv);
+        if (start < h.num_points) {
+            const uint32_t cnt = min(chunk_pts, h.num_points - start);
+            const uint32_t first = h.first_point + start;
+            const int top = best[bi];
+            const uint32_t lead = first & 3u;
+            const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
+            const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
+            const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
+            const u32x4* c4 = reinterpret_cast<const u32x4*>(rgba + (first - lead));
+            const uint32_t nq = (cnt + lead + 3u) >> 2;
+            for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
+                const u32x4 a = __builtin_nontemporal_load(lo4 + q);
+                const u32x4 m = top <= 1 ? __builtin_nontemporal_load(med4 + q) : u32x4{0, 0, 0, 0};
+                const u32x4 c = top == 0 ? __builtin_nontemporal_load(hi4 + q) : u32x4{0, 0, 0, 0};
+                const u32x4 col = PASS == 1 ? __builtin_nontemporal_load(c4 + q) : u32x4{0, 0, 0, 0};
+                for (uint32_t k = 0; k < nv; k++) {
+                    const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
+                    const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
+                    const pcr_view& v = views[vi];
+                    const Decode d = make_decode(lv, h.min, h.max);
+                    uint32_t* dp = depths[vi];
+                    uint32_t* ac = accs[vi];
+                    if (lv == 0) mv_hqs_points4<PASS, 0>(d, a, m, c, col, q * 4u - lead, cnt, v, dp, ac);
+                    else if (lv == 1) mv_hqs_points4<PASS, 1>(d, a, m, c, col, q * 4u - lead, cnt, v, dp, ac);
+                    else mv_hqs_points4<PASS, 2>(d, a, m, c, col, q * 4u - lead, cnt, v, dp, ac);
+                }
+            }
+        }
+        __syncthreads();   // s_item / s_vid are rewritten by the next item
+    }
+}
+
+// window < 0: integer averages with the background (HQS); otherwise the
+// weighted neighbourhood average of NODES_HQS.
+__global__ void k_mv_resolve_hqs(uint32_t* const* __restrict__ depths, uint32_t* const* __restrict__ accs,
+                                 uint32_t* const* __restrict__ out, const pcr_view* __restrict__ views, int window,
+                                 uint32_t bg) {
+    const uint32_t v = blockIdx.y;
+    const int w = views[v].width, h = views[v].height;
+    const uint64_t npix = (uint64_t)w * h;
+    const uint32_t* dp = depths[v];
+    const u32x4* ac = reinterpret_cast<const u32x4*>(accs[v]);
+    uint32_t* o = out[v];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (uint64_t)gridDim.x * blockDim.x)
+        o[i] = window < 0 ? hqs_average(ac[i], bg)
+                          : hqs_window_pixel(dp, ac, w, h, (int)(i % (uint64_t)w), (int)(i / (uint64_t)w), window);
+}
+
 int mv_alloc(MVBuffers& B, int nviews, uint64_t nb) {
     hipError_t e = hipMalloc(&B.d_views, sizeof(pcr_view) * nviews);
     if (e == hipSuccess) e = hipMalloc(&B.d_fbs, sizeof(void*) * nviews);
+    if (e == hipSuccess) e = hipMalloc(&B.d_depths, sizeof(void*) * nviews);
+    if (e == hipSuccess) e = hipMalloc(&B.d_accs, sizeof(void*) * nviews);
     if (e == hipSuccess) e = hipMalloc(&B.d_out, sizeof(void*) * nviews);
     if (e == hipSuccess) e = hipMalloc(&B.d_lod, sizeof(float) * nviews);
     if (e == hipSuccess) e = hipMalloc(&B.d_levels, (nb ? nb : 1) * nviews);
@@ -219,7 +354,8 @@ int mv_alloc(MVBuffers& B, int nviews, uint64_t nb) {
 }
 
 void mv_release(MVBuffers& B) {
-    (void)hipFree(B.d_views); (void)hipFree(B.d_fbs); (void)hipFree(B.d_out); (void)hipFree(B.d_lod);
+    (void)hipFree(B.d_views); (void)hipFree(B.d_fbs); (void)hipFree(B.d_depths); (void)hipFree(B.d_accs);
+    (void)hipFree(B.d_out); (void)hipFree(B.d_lod);
     (void)hipFree(B.d_levels); (void)hipFree(B.d_best); (void)hipFree(B.d_list); (void)hipFree(B.d_state);
     B = MVBuffers{};
 }
@@ -234,11 +370,35 @@ int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nv
     const uint32_t cpb = (maxnp + chunk_pts - 1) / chunk_pts;
     const uint32_t pix_blocks = (uint32_t)std::min<uint64_t>((max_pix + 255) / 256, 4096);
     dim3 g2(pix_blocks, (uint32_t)nviews);
-    k_mv_clear<<<g2, 256, 0, s>>>(B.d_fbs, B.d_views);
+    const bool hqs = prm.mode == PCR_MODE_HQS || prm.mode == PCR_MODE_NODES_HQS;
+    if (hqs) k_mv_hqs_clear<<<g2, 256, 0, s>>>(B.d_depths, B.d_accs, B.d_views);
+    else k_mv_clear<<<g2, 256, 0, s>>>(B.d_fbs, B.d_views);
     k_mv_reset<<<1, 64, 0, s>>>(B.d_state);
     k_mv_select<<<(c->num_batches + 255) / 256, 256, 0, s>>>(c->d_batches, c->num_batches, B.d_views, nviews,
                                                              prm.frustum_culling, lod ? 1 : 0, prm.lod_rule, B.d_lod,
                                                              B.d_levels, B.d_best, B.d_list, B.d_state);
+    if (hqs) {
+#define MV_HQS(BS, PASS)                                                                                        \
+    k_mv_hqs<BS, PASS><<<groups, BS, 0, s>>>(c->d_batches, c->num_batches, B.d_levels, B.d_best, B.d_list,       \
+                                             B.d_state, c->d_lo, c->d_med, c->d_hi, c->d_rgba, B.d_views, nviews, \
+                                             B.d_depths, B.d_accs, chunk_pts, cpb)
+        if (t.tile_px) {
+            int rc = launch_mv_hqs_tiled(s, c, B, nviews, t.tile_px, t.block_size, cu_count);
+            if (rc) return rc;
+        } else {
+            for (int pass = 0; pass < 2; pass++) {
+                if (t.block_size == 64) { if (pass) MV_HQS(64, 1); else MV_HQS(64, 0); }
+                else if (t.block_size == 128) { if (pass) MV_HQS(128, 1); else MV_HQS(128, 0); }
+                else if (t.block_size == 512) { if (pass) MV_HQS(512, 1); else MV_HQS(512, 0); }
+                else { if (pass) MV_HQS(256, 1); else MV_HQS(256, 0); }
+            }
+        }
+#undef MV_HQS
+        const int window = prm.mode == PCR_MODE_NODES_HQS ? prm.resolve_window : -1;
+        k_mv_resolve_hqs<<<g2, 256, 0, s>>>(B.d_depths, B.d_accs, B.d_out, B.d_views, window, prm.background);
+        PCR_HIP(hipGetLastError());
+        return PCR_OK;
+    }
 #define MV_RASTER(BS)                                                                                           \
     k_mv_raster<BS><<<groups, BS, 0, s>>>(c->d_batches, c->num_batches, B.d_levels, B.d_best, B.d_list,          \
                                           B.d_state, c->d_lo, c->d_med, c->d_hi, B.d_views, nviews, B.d_fbs,    \
@@ -296,10 +456,20 @@ int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, 
     PCR_CHECK(ctx && cloud && views && fbs && params && out, "multiview_create: null argument");
     PCR_CHECK(nviews >= 1 && nviews <= kMaxViews, "multiview_create: %d views (1..%d)", nviews, kMaxViews);
     PCR_CHECK(cloud->d_batches, "multiview_create: cloud has no batched layout");
-    PCR_CHECK(params->mode == PCR_MODE_LAS || params->mode == PCR_MODE_LAS2 || params->mode == PCR_MODE_NODES,
-              "multiview_create: mode %d is not a batched key mode (LAS, LAS2, NODES)", params->mode);
+    const int mode = params->mode;
+    const bool hqs = mode == PCR_MODE_HQS || mode == PCR_MODE_NODES_HQS;
+    PCR_CHECK(mode == PCR_MODE_LAS || mode == PCR_MODE_LAS2 || mode == PCR_MODE_NODES || hqs,
+              "multiview_create: mode %d is not a batched mode (LAS, LAS2, NODES, HQS, NODES_HQS)", mode);
+    PCR_CHECK(params->batch_begin == 0 && params->batch_end == 0,
+              "multiview_create: a batch range is not supported (every view draws the whole cloud)");
+    PCR_CHECK(params->resolve_window >= 0 && params->resolve_window <= 4,
+              "multiview_create: resolve_window %d not in 0..4", params->resolve_window);
+    PCR_CHECK(mode != PCR_MODE_NODES && mode != PCR_MODE_NODES_HQS ||
+                  params->lod_rule == PCR_LOD_SPACING || params->lod_rule == PCR_LOD_SIZE80,
+              "multiview_create: bad lod_rule %d", params->lod_rule);
     for (int v = 0; v < nviews; v++) {
-        PCR_CHECK(fbs[v] && fbs[v]->kind == PCR_FB_KEY64, "multiview_create: view %d needs a KEY64 framebuffer", v);
+        PCR_CHECK(fbs[v] && fbs[v]->kind == (hqs ? PCR_FB_HQS : PCR_FB_KEY64),
+                  "multiview_create: view %d needs a %s framebuffer for mode %d", v, hqs ? "HQS" : "KEY64", mode);
         PCR_CHECK(views[v].width == fbs[v]->width && views[v].height == fbs[v]->height,
                   "multiview_create: view %d is %dx%d but its framebuffer is %dx%d", v, views[v].width,
                   views[v].height, fbs[v]->width, fbs[v]->height);
@@ -316,13 +486,15 @@ int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, 
               "multiview_create: tile_px %d needs 1024/2048/4096 with block 128 or 256", t.tile_px);
     pcr_multiview* mv = new pcr_multiview();
     mv->ctx = ctx; mv->cloud = cloud; mv->nviews = nviews; mv->prm = *params; mv->tune = t;
-    mv->lod = params->mode == PCR_MODE_NODES;
+    mv->lod = mode == PCR_MODE_NODES || mode == PCR_MODE_NODES_HQS;
     mv->h_views.assign(views, views + nviews);
     std::vector<unsigned long long*> hf(nviews);
-    std::vector<uint32_t*> ho(nviews);
+    std::vector<uint32_t*> ho(nviews), hd(nviews), ha(nviews);
     std::vector<float> hl(nviews, params->lod_px);
     for (int v = 0; v < nviews; v++) {
         hf[v] = fbs[v]->d_keys;
+        hd[v] = fbs[v]->d_depth;
+        ha[v] = fbs[v]->d_acc;
         ho[v] = fbs[v]->d_rgba;
         mv->max_pix = std::max<uint64_t>(mv->max_pix, (uint64_t)views[v].width * views[v].height);
     }
@@ -331,6 +503,8 @@ int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, 
     hipStream_t s = ctx->stream;
     PCR_HIP(hipMemcpyAsync(mv->B.d_views, views, sizeof(pcr_view) * nviews, hipMemcpyHostToDevice, s));
     PCR_HIP(hipMemcpyAsync(mv->B.d_fbs, hf.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
+    PCR_HIP(hipMemcpyAsync(mv->B.d_depths, hd.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
+    PCR_HIP(hipMemcpyAsync(mv->B.d_accs, ha.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
     PCR_HIP(hipMemcpyAsync(mv->B.d_out, ho.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
     PCR_HIP(hipMemcpyAsync(mv->B.d_lod, hl.data(), sizeof(float) * nviews, hipMemcpyHostToDevice, s));
     PCR_HIP(hipStreamSynchronize(s));

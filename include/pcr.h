@@ -115,9 +115,11 @@ typedef struct pcr_render_params {
     int32_t  block_size;        /* raster workgroup size: 0 = default (256)        */
     int32_t  clear_before;      /* 1: clear the framebuffer before rasterising     */
     int32_t  time_stages;       /* 1: record HIP events around each stage          */
-    int32_t  tile_px;           /* LAS2/NODES: batches whose screen rectangle fits
-                                   in this many pixels merge in an LDS tile first
-                                   (0 = off, else 1024/2048/4096); same result   */
+    int32_t  tile_px;           /* LAS2/NODES/HQS/NODES_HQS: batches whose screen
+                                   rectangle fits in this many pixels merge in an
+                                   LDS tile first (0 = off, else 1024/2048/4096);
+                                   same result. For hqs it also merges the colour
+                                   sums, which is what makes dense views fast    */
     uint32_t batch_begin;       /* draw batches [batch_begin, batch_end) only;     */
     uint32_t batch_end;         /* 0, 0 = all. One LAS file of a multi-file cloud
                                    is drawn with its own view (compute_loop_las.h:
@@ -386,9 +388,14 @@ typedef struct pcr_tune {
 } pcr_tune;
 
 typedef struct pcr_multiview pcr_multiview;
-/* Prepare one launch that renders `nviews` views of `cloud` into `fbs`, walking
- * a (view, batch) work queue; the whole frame (clear, cull, raster, resolve) is
- * captured in a hipGraph when use_graph != 0. */
+/* Prepare one launch that renders `nviews` (1..64) views of `cloud` into `fbs`,
+ * walking a (batch, chunk) work queue that reads each batch once for all the
+ * views that draw it; the whole frame (clear, cull, raster, resolve) is
+ * captured in a hipGraph when use_graph != 0. Modes LAS, LAS2 and NODES take
+ * KEY64 framebuffers; HQS and NODES_HQS take HQS framebuffers and run both hqs
+ * passes over all views, the shared-decode form of the two-eye VR methods
+ * (compute_loop_las_hqs_vr/depth.cs:281-413, color.cs:294-513). Every view's
+ * buffers equal a single-view pcr_render + pcr_resolve of that view. */
 int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nviews,
                          pcr_fb* const* fbs, const pcr_render_params* params,
                          const pcr_tune* tune, int use_graph, pcr_multiview** out);

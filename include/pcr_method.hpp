@@ -3,12 +3,15 @@
 // Mirrors include/Method.h:10-24 (struct Method with name/description/group
 // and per-frame update/render), include/Runtime.h:17-50 (static method
 // registry, selection by name) and the headless part of Renderer::views
-// (include/Renderer.h:78-82). The four concrete methods stand in for
+// (include/Renderer.h:78-82). The concrete methods stand in for
 // ComputeLoopLas (modules/compute_loop_las/compute_loop_las.h), ComputeLoopLas2
-// (compute_loop_las2.h), ComputeLoopLasHqs (compute_loop_las_hqs.h) and
-// ComputeLoopNodes (compute_loop_nodes.h): each render() rasterises the loaded
-// cloud for views[0] and resolves into that view's RGBA8 image. Errors throw
-// pcr_host::Error (the reference skipped silently, compute_loop_las.h:123,210).
+// (compute_loop_las2.h), ComputeLoopLasHqs (compute_loop_las_hqs.h),
+// ComputeLoopNodes (compute_loop_nodes.h) and ComputeLoopNodesHqs
+// (compute_loop_nodes_hqs.h): each render() rasterises the loaded cloud for
+// views[0] and resolves into that view's RGBA8 image. The two VR methods
+// (compute_loop_las_hqs_vr.h, compute_loop_nodes_hqs_vr.h) render views[0]
+// and views[1] together. Errors throw pcr_host::Error (the reference skipped
+// silently, compute_loop_las.h:123,210).
 #pragma once
 #include <cstdint>
 #include <fstream>
@@ -138,6 +141,9 @@ class BatchedMethod : public Method {
         params_.lod_px = 1.0f;
         params_.background = PCR_BG_LAS;
         params_.clear_before = 1;
+        // the LDS-tiled kernels give the same images faster (loop_las keeps
+        // its one-point-per-thread kernel)
+        params_.tile_px = mode == PCR_MODE_LAS ? 0 : 2048;
     }
     ~BatchedMethod() override { pcr_fb_destroy(fb_); }
     void update(Renderer*) override {}
@@ -224,6 +230,89 @@ struct ComputeLoopNodesHqs : BatchedMethod {  // compute_loop_nodes_hqs.h:82-92
         : BatchedMethod(c, cl, one_part(o), PCR_MODE_NODES_HQS, PCR_FB_HQS) {
         name = "loop_nodes_hqs";
         description = "LOD octree nodes with high-quality shading and a weighted resolve";
+        group = "render Potree nodes";
+    }
+};
+
+// The two-eye VR methods (compute_loop_las_hqs_vr.h, compute_loop_nodes_hqs_vr.h)
+// render views[0] and views[1] with one decode of each batch: here one
+// pcr_multiview plan over both views, captured in a hipGraph and rebuilt
+// when a view's size changes. Empty pixels resolve to 0, as the VR resolve
+// writes them (compute_loop_las_hqs_vr/resolve.cs:66-72). Controller input
+// and the world_vr transform stay with the caller: fold them into the views.
+class StereoHqsMethod : public Method {
+  public:
+    StereoHqsMethod(pcr_ctx* ctx, pcr_cloud* cloud, const double origin[3], int mode)
+        : ctx_(ctx), cloud_(cloud), mode_(mode) {
+        for (int i = 0; i < 3; i++) origin_[i] = origin ? origin[i] : 0.0;
+        params_ = pcr_render_params{};
+        params_.mode = mode;
+        params_.frustum_culling = 1;
+        params_.lod_rule = PCR_LOD_SPACING;
+        params_.lod_px = 1.0f;
+        params_.background = 0;
+        params_.resolve_window = 1;
+    }
+    ~StereoHqsMethod() override { release(); }
+    void update(Renderer*) override {}
+    void render(Renderer* r) override {
+        if (!cloud_ || r->views.size() < 2) return;
+        pcr_view pv[2];
+        for (int e = 0; e < 2; e++) {
+            const View& v = r->views[e];
+            check(pcr_view_from_matrices(v.view, v.proj, origin_, v.width, v.height, &pv[e]));
+        }
+        if (!mv_ || w_[0] != pv[0].width || h_[0] != pv[0].height || w_[1] != pv[1].width ||
+            h_[1] != pv[1].height) {
+            release();
+            for (int e = 0; e < 2; e++) {
+                check(pcr_fb_create(ctx_, pv[e].width, pv[e].height, PCR_FB_HQS, &fb_[e]));
+                w_[e] = pv[e].width;
+                h_[e] = pv[e].height;
+            }
+            pcr_tune t{256, 40, 0, 2048};
+            check(pcr_multiview_create(ctx_, cloud_, pv, 2, fb_, &params_, &t, 1, &mv_));
+        } else {
+            check(pcr_multiview_set_views(mv_, pv, 2));
+        }
+        check(pcr_multiview_launch(mv_));
+        for (int e = 0; e < 2; e++) {
+            View& v = r->views[e];
+            v.image.resize((size_t)v.width * v.height);
+            check(pcr_fb_readback_rgba(ctx_, fb_[e], v.image.data()));
+        }
+    }
+    pcr_render_params& params() { return params_; }   // takes effect when the plan is next rebuilt
+    pcr_fb* framebuffer(int eye) { return fb_[eye]; }
+
+  private:
+    void release() {
+        pcr_multiview_destroy(mv_);
+        mv_ = nullptr;
+        for (int e = 0; e < 2; e++) { pcr_fb_destroy(fb_[e]); fb_[e] = nullptr; w_[e] = h_[e] = 0; }
+    }
+    pcr_ctx* ctx_;
+    pcr_cloud* cloud_;
+    int mode_;
+    double origin_[3];
+    pcr_render_params params_;
+    pcr_multiview* mv_ = nullptr;
+    pcr_fb* fb_[2] = {nullptr, nullptr};
+    int w_[2] = {0, 0}, h_[2] = {0, 0};
+};
+
+struct ComputeLoopLasHqsVR : StereoHqsMethod {     // compute_loop_las_hqs_vr.h:61-70
+    ComputeLoopLasHqsVR(pcr_ctx* c, pcr_cloud* cl, const double o[3]) : StereoHqsMethod(c, cl, o, PCR_MODE_HQS) {
+        name = "loop_las_hqs_vr";
+        description = "hqs for two eyes: each batch is decoded once and drawn into both views";
+        group = "10-10-10 bit encoded";
+    }
+};
+struct ComputeLoopNodesHqsVR : StereoHqsMethod {   // compute_loop_nodes_hqs_vr.h:99-108
+    ComputeLoopNodesHqsVR(pcr_ctx* c, pcr_cloud* cl, const double o[3])
+        : StereoHqsMethod(c, cl, o, PCR_MODE_NODES_HQS) {
+        name = "loop_nodes_hqs_vr";
+        description = "LOD nodes with hqs for two eyes, one decode per node, weighted resolve";
         group = "render Potree nodes";
     }
 };

@@ -1,7 +1,8 @@
-// Drives the four Method adapters of include/pcr_method.hpp the way the
+// Drives the Method adapters of include/pcr_method.hpp the way the
 // reference's main loop does (Runtime::setSelectedMethod, then update() and
 // render() per frame, src/main.cpp:342,455) and checks every resolved image
-// against a direct pcr_render + pcr_resolve of the same view.
+// against a direct pcr_render + pcr_resolve of the same view (both eyes for
+// the two VR methods).
 // Prints "OK <method> <drawn pixels>" per method; exit status 0 on success.
 #include <cmath>
 #include <cstdio>
@@ -143,6 +144,8 @@ int main(int argc, char** argv) {
         Runtime::addMethod(new ComputeLoopLasHqs(ctx, cloud, origin));
         Runtime::addMethod(new ComputeLoopNodes(ctx, tree, origin));
         Runtime::addMethod(new ComputeLoopNodesHqs(ctx, tree, origin));
+        Runtime::addMethod(new ComputeLoopLasHqsVR(ctx, cloud, origin));
+        Runtime::addMethod(new ComputeLoopNodesHqsVR(ctx, tree, origin));
 
         Renderer r;
         r.ctx = ctx;
@@ -181,6 +184,49 @@ int main(int argc, char** argv) {
             const bool same = img == v.image;
             std::printf("%s %s %zu\n", same && drawn > 1000 ? "OK" : "FAIL", name, drawn);
             failures += !(same && drawn > 1000);
+        }
+        // the two-eye methods: views[1] is a second eye 6 m to the right
+        r.views.resize(2);
+        r.views[1] = r.views[0];
+        {
+            const double eye1[3] = {1006, -200, 400}, tgt1[3] = {1006, 1000, 50};
+            look_at(eye1, tgt1, up, r.views[1].view);
+        }
+        for (const char* name : {"loop_las_hqs_vr", "loop_nodes_hqs_vr"}) {
+            Runtime::setSelectedMethod(name);
+            Method* m = Runtime::getSelectedMethod();
+            if (!m || m->name != name) { std::printf("FAIL %s not registered\n", name); return 1; }
+            auto* sm = static_cast<StereoHqsMethod*>(m);
+            const int md = sm->params().mode;
+            pcr_cloud* c = md == PCR_MODE_NODES_HQS ? tree : cloud;
+            for (int frame = 0; frame < 2; frame++) {   // the second frame replays the captured plan
+                m->update(&r);
+                m->render(&r);
+                for (int e = 0; e < 2; e++) {
+                    View& ve = r.views[e];
+                    pcr_view pv;
+                    check(pcr_view_from_matrices(ve.view, ve.proj, origin, ve.width, ve.height, &pv));
+                    pcr_fb* fb = nullptr;
+                    check(pcr_fb_create(ctx, ve.width, ve.height, PCR_FB_HQS, &fb));
+                    pcr_render_params p = sm->params();
+                    p.clear_before = 1;
+                    check(pcr_render(ctx, c, &pv, &p, fb));
+                    check(pcr_resolve(ctx, c, &p, fb));
+                    std::vector<uint32_t> img((size_t)ve.width * ve.height);
+                    check(pcr_fb_readback_rgba(ctx, fb, img.data()));
+                    pcr_fb_destroy(fb);
+                    size_t drawn = 0;
+                    for (uint32_t px : ve.image) drawn += px != 0u;
+                    const bool same = img == ve.image;
+                    std::printf("%s %s frame %d eye %d %zu\n", same && drawn > 1000 ? "OK" : "FAIL", name, frame, e,
+                                drawn);
+                    failures += !(same && drawn > 1000);
+                }
+            }
+            if (r.views[0].image == r.views[1].image) {
+                std::printf("FAIL %s: both eyes got the same image\n", name);
+                failures++;
+            }
         }
         for (Method* m : Runtime::methods) delete m;
         pcr_cloud_destroy(tree);

@@ -1,6 +1,6 @@
 """GPU: the C++ Method adapter (include/pcr_method.hpp) driven through the
 reference-style Runtime registry; each method's image equals a direct
-C-ABI render of the same view."""
+C-ABI render of the same view (each eye's, for the two-eye VR methods)."""
 import os
 import subprocess
 
@@ -17,7 +17,9 @@ def test_cpp_methods_render_through_runtime():
     print(out.stdout, out.stderr)
     assert out.returncode == 0, out.stdout + out.stderr
     names = [l.split()[1] for l in out.stdout.splitlines() if l.startswith("OK ")]
-    assert names == ["loop_las", "loop_las_prefetch", "loop_las_hqs", "loop_nodes", "loop_nodes_hqs"]
+    single = ["loop_las", "loop_las_prefetch", "loop_las_hqs", "loop_nodes", "loop_nodes_hqs"]
+    # the two-eye methods print one line per frame and eye (2 frames x 2 eyes)
+    assert names == single + ["loop_las_hqs_vr"] * 4 + ["loop_nodes_hqs_vr"] * 4
 
 
 def test_cpp_las_files_through_loop_las_prefetch(tmp_path):

@@ -64,7 +64,8 @@ def test_multiview_validation(ctx, small):
     v = pcr.look_at((1000, -200, 400), (1000, 1000, 50), up=(0, 0, 1), near=1, far=5000, width=64, height=64)
     fbs = [ctx.framebuffer(64, 64) for _ in range(65)]
     expect_invalid(lambda: pcr.MultiView(ctx, cloud, [v] * 65, fbs, pcr.params(pcr.MODE_LAS2)), "views")
-    expect_invalid(lambda: pcr.MultiView(ctx, cloud, [v], fbs[:1], pcr.params(pcr.MODE_HQS)), "mode")
+    expect_invalid(lambda: pcr.MultiView(ctx, cloud, [v], fbs[:1], pcr.params(pcr.MODE_HQS16)), "mode")
+    expect_invalid(lambda: pcr.MultiView(ctx, cloud, [v], fbs[:1], pcr.params(pcr.MODE_HQS)), "HQS framebuffer")
     expect_invalid(lambda: pcr.MultiView(ctx, cloud, [v], fbs[:1], pcr.params(pcr.MODE_LAS2),
                                          tune=pcr.Tune(512, 40, 0, 2048)), "tile_px")
     mv = pcr.MultiView(ctx, cloud, [v, v], fbs[:2], pcr.params(pcr.MODE_LAS2))

@@ -20,12 +20,13 @@ def octree(ctx):
     cloud.close()
 
 
-@pytest.mark.parametrize("rule,window", [(pcr.LOD_SPACING, 0), (pcr.LOD_SPACING, 1), (pcr.LOD_SIZE80, 2)])
-def test_nodes_hqs_matches_oracle(ctx, octree, rule, window):
+@pytest.mark.parametrize("rule,window,tile", [(pcr.LOD_SPACING, 0, 0), (pcr.LOD_SPACING, 1, 0), (pcr.LOD_SIZE80, 2, 0),
+                                              (pcr.LOD_SPACING, 1, 2048), (pcr.LOD_SIZE80, 2, 1024)])
+def test_nodes_hqs_matches_oracle(ctx, octree, rule, window, tile):
     cloud, lay = octree
     v = pcr.look_at((1000, -300, 500), (1000, 1000, 0), up=(0, 0, 1), near=1, far=5000, width=640, height=480)
     fb = ctx.framebuffer(v.width, v.height, pcr.FB_HQS)
-    prm = pcr.params(pcr.MODE_NODES_HQS, lod_rule=rule, lod_px=1.0, resolve_window=window)
+    prm = pcr.params(pcr.MODE_NODES_HQS, lod_rule=rule, lod_px=1.0, resolve_window=window, tile_px=tile)
     ctx.render(cloud, v, prm, fb)
     ctx.resolve(cloud, prm, fb)
     depth, acc = fb.hqs()

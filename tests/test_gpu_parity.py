@@ -108,10 +108,12 @@ def test_precision_levels_all_used(ctx, terrain):
     assert len(set(levels.tolist()) - {255}) >= 2
 
 
-def test_hqs_batched(ctx, terrain):
+@pytest.mark.parametrize("tile,block", [(0, 0), (2048, 256), (1024, 128), (4096, 256)])
+def test_hqs_batched(ctx, terrain, tile, block):
+    """Direct and LDS-tiled hqs passes against the oracle."""
     cloud, lay = terrain
     fb = ctx.framebuffer(640, 360, pcr.FB_HQS)
-    prm = pcr.params(pcr.MODE_HQS)
+    prm = pcr.params(pcr.MODE_HQS, tile_px=tile, block_size=block)
     for view in terrain_views(n=2, seed=9):
         ctx.render(cloud, view, prm, fb)
         ctx.resolve(cloud, prm, fb)
@@ -120,6 +122,26 @@ def test_hqs_batched(ctx, terrain):
         assert np.array_equal(d, rd)
         assert np.array_equal(a, ra)
         assert np.array_equal(fb.rgba(), orc.hqs_resolve(ra, prm.background))
+
+
+@pytest.mark.parametrize("tile", [0, 2048])
+def test_hqs_dense_far_view(ctx, terrain, tile):
+    """The whole terrain from far away at 160x90: each pixel gathers
+    hundreds of points within 1.01x of its depth, so the tiled pass 2 merges
+    many colours per pixel in LDS before its global adds."""
+    cloud, lay = terrain
+    view = pcr.look_at((1000, -6000, 3000), (1000, 1000, 0), up=(0, 0, 1), near=1.0, far=20000.0, width=160,
+                       height=90)
+    fb = ctx.framebuffer(160, 90, pcr.FB_HQS)
+    prm = pcr.params(pcr.MODE_HQS, tile_px=tile)
+    ctx.render(cloud, view, prm, fb)
+    ctx.resolve(cloud, prm, fb)
+    d, a = fb.hqs()
+    rd, ra = orc.hqs_batched(lay["batches"], lay["lo"], lay["med"], lay["hi"], lay["rgba"], view, 1)
+    assert ra[3::4].max() > 200
+    assert np.array_equal(d, rd) and np.array_equal(a, ra)
+    assert np.array_equal(fb.rgba(), orc.hqs_resolve(ra, prm.background))
+    fb.close()
 
 
 def test_hqs16_config3(ctx):

@@ -79,12 +79,13 @@ def test_unaligned_batches_match_oracle(ctx, layout, mode, tile):
 
 
 @pytest.mark.gpu
-def test_unaligned_batches_hqs(ctx, layout):
+@pytest.mark.parametrize("tile", [0, 2048])
+def test_unaligned_batches_hqs(ctx, layout, tile):
     b, lo, med, hi, rgba = layout
     cloud = ctx.upload_batched(b, lo, med, hi, rgba)
     v = cams()[0]
     fb = ctx.framebuffer(v.width, v.height, pcr.FB_HQS)
-    prm = pcr.params(pcr.MODE_HQS)
+    prm = pcr.params(pcr.MODE_HQS, tile_px=tile)
     ctx.render(cloud, v, prm, fb)
     depth, acc = fb.hqs()
     rdepth, racc = orc.hqs_batched(b, lo, med, hi, rgba, v, 1)
