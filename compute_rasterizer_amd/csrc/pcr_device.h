@@ -269,17 +269,32 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void hqs_depth(uint32_t* __restrict__ depth, uint32_t pix, uint32_t d) {
     if (d < depth[pix]) atomicMin(depth + pix, d);
 }
+// Adds (lo0, hi0) to the uint32 sums p[0], p[1] and (lo1, hi1) to p[2], p[3]
+// (p 16-byte aligned) with one 64-bit atomic per pair, keeping every sum
+// independent mod 2^32 as four uint32 atomics would: when a pair's low sum
+// wraps, the 64-bit add carries 1 into its high sum, and a second atomic
+// takes it back. The low sum's running value only grows, so it wraps exactly
+// floor(total / 2^32) times whatever the order of the adds, and each wrap is
+// seen by the one atomic whose returned old value plus lo overflows. The hqs
+// accumulators use it for (R, G) and (B, count): half the atomics of four
+// uint32 adds (the reference packs R | G << 32 the same way,
+// hqs/color.cs:432-436, but lets the carry stand). Both atomics are issued
+// before either return value is checked, so a point waits for one round trip.
+__device__ __forceinline__ void add_pairs(uint32_t* p, uint32_t lo0, uint32_t hi0, uint32_t lo1, uint32_t hi1) {
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+    const unsigned long long old0 = atomicAdd(q, (unsigned long long)lo0 | ((unsigned long long)hi0 << 32));
+    const unsigned long long old1 = atomicAdd(q + 1, (unsigned long long)lo1 | ((unsigned long long)hi1 << 32));
+    if ((uint32_t)((uint32_t)old0 + lo0) < lo0) atomicAdd(q, 0xFFFFFFFF00000000ull);
+    if ((uint32_t)((uint32_t)old1 + lo1) < lo1) atomicAdd(q + 1, 0xFFFFFFFF00000000ull);
+}
+
 // hqs pass 2 (hqs/color.cs:398-437): a point within 1.01x of the pixel's
 // minimum depth adds R, G, B and 1 to the pixel's four uint32 accumulators.
 __device__ __forceinline__ void hqs_accum(const uint32_t* __restrict__ depth, uint32_t* __restrict__ acc,
                                           uint32_t pix, uint32_t d, uint32_t color) {
     const float bd = __uint_as_float(depth[pix]);
     if (!(__uint_as_float(d) <= bd * 1.01f)) return;
-    uint32_t* a = acc + 4ull * pix;
-    atomicAdd(a + 0, color & 255u);
-    atomicAdd(a + 1, (color >> 8) & 255u);
-    atomicAdd(a + 2, (color >> 16) & 255u);
-    atomicAdd(a + 3, 1u);
+    add_pairs(acc + 4ull * pix, color & 255u, (color >> 8) & 255u, (color >> 16) & 255u, 1u);
 }
 // hqs/resolve.cs:74-129: integer averages, background where count == 0.
 __device__ __forceinline__ uint32_t hqs_average(const u32x4 a, uint32_t bg) {

@@ -551,7 +551,7 @@ int launch_mv_raster_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B
 //   pass 2: loads the rectangle's final depths into LDS, accumulates its
 //           visible points there as two packed 64-bit sums per pixel,
 //           R | G << 32 and B | count << 32 (ds_add_u64), then adds each
-//           touched pixel's four sums to the global uint32 accumulators.
+//           touched pixel's sums to the global uint32 accumulators (add_pairs).
 // A batch of at most kHqsTileMaxPoints points cannot carry out of a 32-bit
 // half (255 x 16,843,009 < 2^32), and uint32 addition is associative mod
 // 2^32, so depth and accumulators are bit-identical to the direct passes.
@@ -691,11 +691,7 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
         } else {
             const unsigned long long rg = tacc[2 * i], bc = tacc[2 * i + 1];
             if ((bc >> 32) != 0ull) {
-                uint32_t* a = acc + 4ull * pix;
-                if ((uint32_t)rg) atomicAdd(a + 0, (uint32_t)rg);
-                if ((uint32_t)(rg >> 32)) atomicAdd(a + 1, (uint32_t)(rg >> 32));
-                if ((uint32_t)bc) atomicAdd(a + 2, (uint32_t)bc);
-                atomicAdd(a + 3, (uint32_t)(bc >> 32));
+                add_pairs(acc + 4ull * pix, (uint32_t)rg, (uint32_t)(rg >> 32), (uint32_t)bc, (uint32_t)(bc >> 32));
             }
         }
     }
