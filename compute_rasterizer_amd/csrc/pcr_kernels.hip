@@ -560,33 +560,49 @@ int launch_mv_raster_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B
 // contended global atomics.
 constexpr uint32_t kHqsTileMaxPoints = 16843009u;
 
+// A batch's screen rectangle [x0, x0 + tw) x [y0, y0 + th) and the band of
+// its rows, [y0 + boff, y0 + boff + bh), that the LDS tile holds in this
+// pass over the batch's points. Points in the rectangle but outside the band
+// belong to another band; points outside the rectangle (possible only where
+// the rounding of the rectangle and of a point's pixel disagree) take the
+// direct path once, in the first band.
+struct TileBand {
+    int x0, y0, tw, th, boff, bh;
+    bool first;
+};
+
 template <int PASS, int LC, bool TILE>
 __device__ __forceinline__ void hqs_tile_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi,
                                                uint32_t color, uint32_t local, uint32_t np, const Proj& pj,
                                                uint32_t* __restrict__ depth, uint32_t* __restrict__ acc,
-                                               uint32_t* tdepth, unsigned long long* tacc, int x0, int y0, int tw,
-                                               int th) {
+                                               uint32_t* tdepth, unsigned long long* tacc, const TileBand& tb) {
     if (local >= np) return;
     float p[3];
     decode<LC>(d, lo, med, hi, p);
     int ix, iy;
     uint32_t dep;
     if (!project_xy(pj, p[0], p[1], p[2], ix, iy, dep)) return;
-    const uint32_t ux = (uint32_t)(ix - x0), uy = (uint32_t)(iy - y0);
-    if (TILE && ux < (uint32_t)tw && uy < (uint32_t)th) {
-        const uint32_t t = uy * (uint32_t)tw + ux;
-        if (PASS == 0) {
-            lds_min32(tdepth + t, dep);
-        } else {
-            if (!(__uint_as_float(dep) <= __uint_as_float(tdepth[t]) * 1.01f)) return;
-            lds_add64(tacc + 2 * t, (unsigned long long)(color & 255u) | ((unsigned long long)((color >> 8) & 255u) << 32));
-            lds_add64(tacc + 2 * t + 1, (unsigned long long)((color >> 16) & 255u) | (1ull << 32));
+    if (TILE) {
+        const uint32_t ux = (uint32_t)(ix - tb.x0), ry = (uint32_t)(iy - tb.y0);
+        if (ux < (uint32_t)tb.tw && ry < (uint32_t)tb.th) {
+            const uint32_t by = ry - (uint32_t)tb.boff;
+            if (by >= (uint32_t)tb.bh) return;
+            const uint32_t t = by * (uint32_t)tb.tw + ux;
+            if (PASS == 0) {
+                lds_min32(tdepth + t, dep);
+            } else {
+                if (!(__uint_as_float(dep) <= __uint_as_float(tdepth[t]) * 1.01f)) return;
+                lds_add64(tacc + 2 * t,
+                          (unsigned long long)(color & 255u) | ((unsigned long long)((color >> 8) & 255u) << 32));
+                lds_add64(tacc + 2 * t + 1, (unsigned long long)((color >> 16) & 255u) | (1ull << 32));
+            }
+            return;
         }
-    } else {
-        const uint32_t pix = (uint32_t)ix + (uint32_t)iy * (uint32_t)pj.w;
-        if (PASS == 0) hqs_depth(depth, pix, dep);
-        else hqs_accum(depth, acc, pix, dep, color);
+        if (!tb.first) return;
     }
+    const uint32_t pix = (uint32_t)ix + (uint32_t)iy * (uint32_t)pj.w;
+    if (PASS == 0) hqs_depth(depth, pix, dep);
+    else hqs_accum(depth, acc, pix, dep, color);
 }
 
 template <int BLOCK, int PASS, int LC, bool TILE, bool NT>
@@ -595,7 +611,7 @@ __device__ __forceinline__ void hqs_tile_loop(const Decode& d, const uint32_t* _
                                               const uint32_t* __restrict__ rgba, uint32_t first, uint32_t np,
                                               const Proj& pj, uint32_t* __restrict__ depth,
                                               uint32_t* __restrict__ acc, uint32_t* tdepth, unsigned long long* tacc,
-                                              int x0, int y0, int tw, int th) {
+                                              const TileBand& tb) {
     const uint32_t lead = first & 3u;   // see tile_loop()
     const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
     const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
@@ -608,10 +624,10 @@ __device__ __forceinline__ void hqs_tile_loop(const Decode& d, const uint32_t* _
         const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 col = PASS == 1 ? ld_plane<NT>(c4 + q) : u32x4{0, 0, 0, 0};
         const uint32_t base = q * 4u - lead;
-        hqs_tile_point<PASS, LC, TILE>(d, a.x, m.x, c.x, col.x, base + 0, np, pj, depth, acc, tdepth, tacc, x0, y0, tw, th);
-        hqs_tile_point<PASS, LC, TILE>(d, a.y, m.y, c.y, col.y, base + 1, np, pj, depth, acc, tdepth, tacc, x0, y0, tw, th);
-        hqs_tile_point<PASS, LC, TILE>(d, a.z, m.z, c.z, col.z, base + 2, np, pj, depth, acc, tdepth, tacc, x0, y0, tw, th);
-        hqs_tile_point<PASS, LC, TILE>(d, a.w, m.w, c.w, col.w, base + 3, np, pj, depth, acc, tdepth, tacc, x0, y0, tw, th);
+        hqs_tile_point<PASS, LC, TILE>(d, a.x, m.x, c.x, col.x, base + 0, np, pj, depth, acc, tdepth, tacc, tb);
+        hqs_tile_point<PASS, LC, TILE>(d, a.y, m.y, c.y, col.y, base + 1, np, pj, depth, acc, tdepth, tacc, tb);
+        hqs_tile_point<PASS, LC, TILE>(d, a.z, m.z, c.z, col.z, base + 2, np, pj, depth, acc, tdepth, tacc, tb);
+        hqs_tile_point<PASS, LC, TILE>(d, a.w, m.w, c.w, col.w, base + 3, np, pj, depth, acc, tdepth, tacc, tb);
     }
 }
 
@@ -621,16 +637,16 @@ __device__ __forceinline__ void hqs_tile_dispatch(int level, const Decode& d, co
                                                   const uint32_t* __restrict__ rgba, const pcr_batch& h,
                                                   const Proj& pj, uint32_t* __restrict__ depth,
                                                   uint32_t* __restrict__ acc, uint32_t* tdepth,
-                                                  unsigned long long* tacc, int x0, int y0, int tw, int th) {
+                                                  unsigned long long* tacc, const TileBand& tb) {
     if (level == 0)
         hqs_tile_loop<BLOCK, PASS, 0, TILE, NT>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth, acc,
-                                                tdepth, tacc, x0, y0, tw, th);
+                                                tdepth, tacc, tb);
     else if (level == 1)
         hqs_tile_loop<BLOCK, PASS, 1, TILE, NT>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth, acc,
-                                                tdepth, tacc, x0, y0, tw, th);
+                                                tdepth, tacc, tb);
     else
         hqs_tile_loop<BLOCK, PASS, 2, TILE, NT>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth, acc,
-                                                tdepth, tacc, x0, y0, tw, th);
+                                                tdepth, tacc, tb);
 }
 
 // Pass 2 keeps 20 B per tile pixel; its tile holds 8 pixels fewer than
@@ -644,7 +660,14 @@ struct HqsTile {
 };
 
 // One batch of one hqs pass; `bi` and `level` are workgroup-uniform, and so
-// is `use` (every thread derives it from the same header and view).
+// are the rectangle and the band count (every thread derives them from the
+// same header and view). A rectangle of at most the tile's width is cut
+// into bands of whole rows that each fit the tile; up to kHqsMaxBands bands
+// re-read and re-project the batch's points once per band, which costs far
+// less than the scattered global atomics of the direct path, and each
+// touched pixel is flushed once, in coalesced rows.
+constexpr int kHqsMaxBands = 8;
+
 template <int BLOCK, int TILE_PIX, int PASS, bool NT>
 __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
                                                 const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
@@ -656,42 +679,49 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
     const Decode d = make_decode(level, h.min, h.max);
     const Proj pj = make_proj(v.transform, v.width, v.height);
     int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
-    bool use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1);
+    const bool on_screen = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1);
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
     constexpr int cap = HqsTile<PASS, TILE_PIX>::kCap;
-    use = use && tw <= cap && th <= cap && tw * th <= cap && h.num_points <= kHqsTileMaxPoints;
-    if (!use) {
-        hqs_tile_dispatch<BLOCK, PASS, false, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, nullptr, nullptr, 0,
-                                                  0, 0, 0);
+    const bool fits = on_screen && tw <= cap && h.num_points <= kHqsTileMaxPoints;
+    const int bh = fits ? min(th, cap / tw) : 1;    // rows per band
+    const int nbands = fits ? (th + bh - 1) / bh : 0;
+    if (!fits || nbands > kHqsMaxBands) {
+        const TileBand none{0, 0, 0, 0, 0, 0, true};
+        hqs_tile_dispatch<BLOCK, PASS, false, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, nullptr, nullptr,
+                                                  none);
         return;
     }
-    const int n = tw * th;
     // row = i / tw as a multiply-high (see tiled_batch)
     const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
-    for (int i = threadIdx.x; i < n; i += BLOCK) {
-        if (PASS == 0) {
-            tdepth[i] = PCR_DEPTH_CLEAR;
-        } else {
-            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
-            tdepth[i] = depth[(uint32_t)x0 + ux + ((uint32_t)y0 + uy) * (uint32_t)v.width];
-            tacc[2 * i] = 0ull;
-            tacc[2 * i + 1] = 0ull;
+    for (int k = 0; k < nbands; k++) {
+        const int boff = k * bh, rows = min(bh, th - boff);
+        const int n = tw * rows;
+        const uint32_t by0 = (uint32_t)(y0 + boff);
+        if (k) __syncthreads();   // the previous band's flush has read the tile
+        for (int i = threadIdx.x; i < n; i += BLOCK) {
+            if (PASS == 0) {
+                tdepth[i] = PCR_DEPTH_CLEAR;
+            } else {
+                const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
+                tdepth[i] = depth[(uint32_t)x0 + ux + (by0 + uy) * (uint32_t)v.width];
+                tacc[2 * i] = 0ull;
+                tacc[2 * i + 1] = 0ull;
+            }
         }
-    }
-    __syncthreads();
-    hqs_tile_dispatch<BLOCK, PASS, true, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, tdepth, tacc, x0, y0, tw,
-                                             th);
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += BLOCK) {
-        const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
-        const uint32_t pix = (uint32_t)x0 + ux + ((uint32_t)y0 + uy) * (uint32_t)v.width;
-        if (PASS == 0) {
-            const uint32_t t = tdepth[i];
-            if (t != PCR_DEPTH_CLEAR) hqs_depth(depth, pix, t);
-        } else {
-            const unsigned long long rg = tacc[2 * i], bc = tacc[2 * i + 1];
-            if ((bc >> 32) != 0ull) {
-                add_pairs(acc + 4ull * pix, (uint32_t)rg, (uint32_t)(rg >> 32), (uint32_t)bc, (uint32_t)(bc >> 32));
+        __syncthreads();
+        const TileBand tb{x0, y0, tw, th, boff, rows, k == 0};
+        hqs_tile_dispatch<BLOCK, PASS, true, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, tdepth, tacc, tb);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += BLOCK) {
+            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
+            const uint32_t pix = (uint32_t)x0 + ux + (by0 + uy) * (uint32_t)v.width;
+            if (PASS == 0) {
+                const uint32_t t = tdepth[i];
+                if (t != PCR_DEPTH_CLEAR) hqs_depth(depth, pix, t);
+            } else {
+                const unsigned long long rg = tacc[2 * i], bc = tacc[2 * i + 1];
+                if ((bc >> 32) != 0ull)
+                    add_pairs(acc + 4ull * pix, (uint32_t)rg, (uint32_t)(rg >> 32), (uint32_t)bc, (uint32_t)(bc >> 32));
             }
         }
     }
