@@ -33,14 +33,16 @@ def look_at(eye, target, up=(0.0, 1.0, 0.0), fovy=60.0, near=0.1, far=1000.0, or
 
 
 def params(mode=MODE_LAS2, culling=1, background=None, lod_rule=LOD_SPACING, lod_px=1.0, block_size=0,
-           clear_before=1, time_stages=0, tile_px=None, batch_begin=0, batch_end=0, resolve_window=0) -> RenderParams:
-    """Render parameters; LAS2 and NODES default to the LDS-tiled raster (2048-px tile)."""
+           clear_before=1, time_stages=0, tile_px=None, batch_begin=0, batch_end=0, resolve_window=0,
+           edl_strength=0.0) -> RenderParams:
+    """Render parameters; LAS2, NODES, HQS and NODES_HQS default to the LDS-tiled
+    kernels (2048-px tile)."""
     if background is None:
         background = BG_BLACK if mode in (MODE_PLAIN16,) else BG_LAS
     if tile_px is None:
-        tile_px = 2048 if mode in (MODE_LAS2, MODE_NODES) else 0
+        tile_px = 2048 if mode in (MODE_LAS2, MODE_NODES, MODE_HQS, MODE_NODES_HQS) else 0
     return RenderParams(mode, culling, lod_rule, lod_px, background, block_size, clear_before, time_stages,
-                        tile_px, batch_begin, batch_end, resolve_window)
+                        tile_px, batch_begin, batch_end, resolve_window, edl_strength)
 
 
 def potree_info(metadata_json: str) -> capi.PotreeInfo:

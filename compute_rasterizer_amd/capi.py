@@ -60,7 +60,8 @@ class RenderParams(C.Structure):
     _fields_ = [("mode", C.c_int32), ("frustum_culling", C.c_int32), ("lod_rule", C.c_int32),
                 ("lod_px", C.c_float), ("background", C.c_uint32), ("block_size", C.c_int32),
                 ("clear_before", C.c_int32), ("time_stages", C.c_int32), ("tile_px", C.c_int32),
-                ("batch_begin", C.c_uint32), ("batch_end", C.c_uint32), ("resolve_window", C.c_int32)]
+                ("batch_begin", C.c_uint32), ("batch_end", C.c_uint32), ("resolve_window", C.c_int32),
+                ("edl_strength", C.c_float)]
 
 
 class LasInfo(C.Structure):

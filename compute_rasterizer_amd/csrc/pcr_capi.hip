@@ -486,10 +486,16 @@ int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params,
     PCR_CHECK(hqs == (fb->kind == PCR_FB_HQS), "resolve: mode %d does not match the framebuffer kind", mode);
     PCR_CHECK(params->resolve_window >= 0 && params->resolve_window <= 4, "resolve: resolve_window %d not in 0..4",
               params->resolve_window);
+    PCR_CHECK(params->edl_strength == 0.0f ||
+                  (params->edl_strength > 0.0f && params->edl_strength < 1.0e30f &&
+                   (mode == PCR_MODE_LAS || mode == PCR_MODE_LAS2 || mode == PCR_MODE_NODES)),
+              "resolve: edl_strength %g needs a finite positive value and an index mode (LAS, LAS2, NODES)",
+              (double)params->edl_strength);
     PCR_CHECK(hqs || mode == PCR_MODE_PLAIN16 || cloud->d_rgba, "resolve: cloud has no colour plane");
     hipEvent_t* ev = ctx->ring[(ctx->frames_timed + PCR_TIMING_RING - 1) % PCR_TIMING_RING];
     if (ctx->timed) PCR_HIP(hipEventRecord(ev[4], ctx->stream));
-    int rc = launch_resolve(ctx->stream, cloud, mode, params->background, params->resolve_window, fb);
+    int rc = launch_resolve(ctx->stream, cloud, mode, params->background, params->resolve_window,
+                            params->edl_strength, fb);
     if (rc) return rc;
     if (ctx->timed) PCR_HIP(hipEventRecord(ev[5], ctx->stream));
     return PCR_OK;

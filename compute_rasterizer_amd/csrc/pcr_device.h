@@ -336,4 +336,66 @@ __device__ inline uint32_t hqs_window_pixel(const uint32_t* __restrict__ depth, 
     return color;
 }
 
+// e^x for x <= 0, for the eye-dome lighting shade. The oracle runs the same
+// sequence (orc_edl_exp), so the shade has the same bits on both sides.
+// Method: Cody-Waite range reduction x = k ln2 + r, with ln2 split into a
+// high part of 17 significant bits (k * ln2_hi is exact for |k| < 128) and a
+// low part; e^r from its Taylor series to r^7 in Horner form (|r| <= ln2/2,
+// truncation error below 2^-27 relative); then scaled by 2^k with ldexp.
+// x <= -86 returns 0: e^-86 < 2^-124, and an 8-bit channel times it
+// truncates to 0 like the true value.
+__device__ inline float edl_exp(float x) {
+    if (!(x > -86.0f)) return 0.0f;
+    const float k = __builtin_rintf(x * 1.44269504f);
+    float r = fma_(-k, 0.693145751953125f, x);
+    r = fma_(-k, 1.42860677e-06f, r);
+    float p = 1.98412698e-04f;           // 1/7!
+    p = fma_(p, r, 1.38888889e-03f);     // 1/6!
+    p = fma_(p, r, 8.33333333e-03f);     // 1/5!
+    p = fma_(p, r, 4.16666667e-02f);     // 1/4!
+    p = fma_(p, r, 1.66666667e-01f);     // 1/3!
+    p = fma_(p, r, 0.5f);
+    p = fma_(p, r, 1.0f);
+    p = fma_(p, r, 1.0f);
+    return __builtin_ldexpf(p, (int)k);
+}
+
+// getDepthAt(pixel, window 0) of compute_loop_las/resolve.cs:43-64: the
+// pixel's depth if it is in (0, 1e8), else 1e8. A neighbour outside the
+// image counts as background (the shader reads past the row or buffer).
+__device__ __forceinline__ float edl_depth(const unsigned long long* __restrict__ fb, int w, int h, int x, int y) {
+    if (x < 0 || y < 0 || x >= w || y >= h) return 100000000.0f;
+    const float d = __uint_as_float((uint32_t)(fb[(uint64_t)y * w + x] >> 32));
+    return (d > 0.0f && d < 100000000.0f) ? d : 100000000.0f;
+}
+
+// The colour of the point in pixel (x, y) for an index-payload framebuffer
+// (compute_loop_las/resolve.cs:100-146): bg for a clear key or an index
+// outside the cloud. With strength > 0, the EDL block the shader carries
+// switched off (resolve.cs:148-191): shade = exp(-(sum / 9) * 300 *
+// strength), sum adding max(0, depth - neighbour depth) over the 3x3 window
+// in ox-major order; channels truncate to integers and alpha is 0.
+__device__ inline uint32_t resolve_index_pixel(const unsigned long long* __restrict__ fb, int w, int h, int x, int y,
+                                               const uint32_t* __restrict__ rgba, uint64_t npoints, uint32_t bg,
+                                               float strength) {
+    const unsigned long long key = fb[(uint64_t)y * w + x];
+    const uint32_t id = (uint32_t)key;
+    if (key == PCR_FB_CLEAR || id >= npoints) return bg;
+    const uint32_t color = rgba[id];
+    if (!(strength > 0.0f)) return color;
+    const float depth = __uint_as_float((uint32_t)(key >> 32));
+    float sum = 0.0f;
+    for (int ox = -1; ox <= 1; ox++)
+        for (int oy = -1; oy <= 1; oy++) {
+            const float dd = depth - edl_depth(fb, w, h, x + ox, y + oy);
+            sum += dd > 0.0f ? dd : 0.0f;
+        }
+    const float response = sum / 9.0f;
+    const float shade = edl_exp(-response * 300.0f * strength);
+    const uint32_t r = (uint32_t)((float)(color & 255u) * shade);
+    const uint32_t g = (uint32_t)((float)((color >> 8) & 255u) * shade);
+    const uint32_t b = (uint32_t)((float)((color >> 16) & 255u) * shade);
+    return r | (g << 8) | (b << 16);
+}
+
 }  // namespace pcr

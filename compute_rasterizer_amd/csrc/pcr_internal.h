@@ -97,7 +97,7 @@ int launch_hqs_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, Fra
 int launch_hqs_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block, FrameState* st,
                      pcr_fb* fb);
 int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pcr_fb* fb);
-int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, pcr_fb* fb);
+int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, float edl, pcr_fb* fb);
 int launch_selftest_divide(hipStream_t s, const float* nx, const float* ny, const float* den, uint64_t n,
                            float* out2);
 

@@ -1031,11 +1031,19 @@ int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pc
 // ---------------------------------------------------------------- resolve
 // Index payload: colour gathered from the colour plane (resolve.cs:117-146).
 // An index outside the cloud (a framebuffer filled by another cloud or mode)
-// resolves to the background instead of reading out of bounds.
-__global__ void k_resolve_index(const unsigned long long* __restrict__ fb, uint64_t npix,
-                                const uint32_t* __restrict__ rgba, uint64_t npoints, uint32_t bg,
+// resolves to the background instead of reading out of bounds. strength > 0
+// adds the eye-dome lighting shade (resolve_index_pixel).
+__global__ void k_resolve_index(const unsigned long long* __restrict__ fb, int w, int h,
+                                const uint32_t* __restrict__ rgba, uint64_t npoints, uint32_t bg, float strength,
                                 uint32_t* __restrict__ out) {
+    const uint64_t npix = (uint64_t)w * h;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (strength > 0.0f) {   // npix <= 2^30, so 32-bit division finds the pixel's row
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride)
+            out[i] = resolve_index_pixel(fb, w, h, (int)((uint32_t)i % (uint32_t)w), (int)((uint32_t)i / (uint32_t)w),
+                                         rgba, npoints, bg, strength);
+        return;
+    }
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
         const unsigned long long k = fb[i];
         const uint32_t id = (uint32_t)k;
@@ -1067,7 +1075,7 @@ __global__ void k_resolve_hqs_window(const uint32_t* __restrict__ depth, const u
         out[i] = hqs_window_pixel(depth, acc, w, h, (int)(i % (uint64_t)w), (int)(i / (uint64_t)w), window);
 }
 
-int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, pcr_fb* fb) {
+int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, float edl, pcr_fb* fb) {
     const uint64_t npix = (uint64_t)fb->width * fb->height;
     const int g = grid_for(npix, 256);
     if (mode == PCR_MODE_NODES_HQS) {
@@ -1078,7 +1086,8 @@ int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int
     } else if (mode == PCR_MODE_PLAIN16) {
         k_resolve_color<<<g, 256, 0, s>>>(fb->d_keys, npix, bg, fb->d_rgba);
     } else {
-        k_resolve_index<<<g, 256, 0, s>>>(fb->d_keys, npix, c->d_rgba, c->num_points, bg, fb->d_rgba);
+        k_resolve_index<<<g, 256, 0, s>>>(fb->d_keys, fb->width, fb->height, c->d_rgba, c->num_points, bg, edl,
+                                          fb->d_rgba);
     }
     PCR_HIP(hipGetLastError());
     return PCR_OK;

@@ -191,12 +191,20 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster(const pcr_batch* __restrict
 
 __global__ void k_mv_resolve(const unsigned long long* const* __restrict__ fbs, uint32_t* const* __restrict__ out,
                              const pcr_view* __restrict__ views, const uint32_t* __restrict__ rgba, uint64_t npoints,
-                             uint32_t bg) {
+                             uint32_t bg, float edl) {
     const uint32_t v = blockIdx.y;
-    const uint64_t npix = (uint64_t)views[v].width * views[v].height;
+    const int w = views[v].width, h = views[v].height;
+    const uint64_t npix = (uint64_t)w * h;
     const unsigned long long* fb = fbs[v];
     uint32_t* o = out[v];
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (edl > 0.0f) {   // npix <= 2^30, so 32-bit division finds the pixel's row
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride)
+            o[i] = resolve_index_pixel(fb, w, h, (int)((uint32_t)i % (uint32_t)w), (int)((uint32_t)i / (uint32_t)w),
+                                       rgba, npoints, bg, edl);
+        return;
+    }
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
         const unsigned long long k = fb[i];
         const uint32_t id = (uint32_t)k;
         o[i] = (k == PCR_FB_CLEAR || id >= npoints) ? bg : rgba[id];
@@ -411,7 +419,8 @@ int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nv
     else if (t.block_size == 512) MV_RASTER(512);
     else MV_RASTER(256);
 #undef MV_RASTER
-    k_mv_resolve<<<g2, 256, 0, s>>>(B.d_fbs, B.d_out, B.d_views, c->d_rgba, c->num_points, prm.background);
+    k_mv_resolve<<<g2, 256, 0, s>>>(B.d_fbs, B.d_out, B.d_views, c->d_rgba, c->num_points, prm.background,
+                                    prm.edl_strength);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
 }
@@ -464,6 +473,10 @@ int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, 
               "multiview_create: a batch range is not supported (every view draws the whole cloud)");
     PCR_CHECK(params->resolve_window >= 0 && params->resolve_window <= 4,
               "multiview_create: resolve_window %d not in 0..4", params->resolve_window);
+    PCR_CHECK(params->edl_strength == 0.0f ||
+                  (params->edl_strength > 0.0f && params->edl_strength < 1.0e30f && !hqs),
+              "multiview_create: edl_strength %g needs a finite positive value and an index mode",
+              (double)params->edl_strength);
     PCR_CHECK(mode != PCR_MODE_NODES && mode != PCR_MODE_NODES_HQS ||
                   params->lod_rule == PCR_LOD_SPACING || params->lod_rule == PCR_LOD_SIZE80,
               "multiview_create: bad lod_rule %d", params->lod_rule);

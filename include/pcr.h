@@ -126,6 +126,10 @@ typedef struct pcr_render_params {
                                    141-199 gives each file its own transform)     */
     int32_t  resolve_window;    /* PCR_MODE_NODES_HQS: neighbourhood radius of the
                                    resolve, 0..4 (0 in nodes_hqs/resolve.cs:98)    */
+    float    edl_strength;      /* LAS/LAS2/NODES resolve: eye-dome lighting, the
+                                   shading that compute_loop_las/resolve.cs:
+                                   148-191 carries switched off (its constant is
+                                   0.0005); 0 = off                              */
 } pcr_render_params;
 
 typedef struct pcr_stats {

@@ -55,6 +55,9 @@ def load():
         f.orc_render_batched.argtypes = [P, C.c_uint32, P, P, P, C.POINTER(View), C.c_int, P, P]
         f.orc_resolve_index.argtypes = [P, C.c_uint64, P, C.c_uint32, P]
         f.orc_resolve_color.argtypes = [P, C.c_uint64, C.c_uint32, P]
+        f.orc_edl_exp.argtypes = [C.c_float]
+        f.orc_edl_exp.restype = C.c_float
+        f.orc_resolve_edl.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_uint32, C.c_float, P]
         f.orc_hqs_batched.argtypes = [P, C.c_uint32, P, P, P, P, C.POINTER(View), C.c_int, P, P]
         f.orc_hqs_points16.argtypes = [P, P, C.c_uint64, C.POINTER(View), P, P]
         f.orc_hqs_resolve.argtypes = [P, C.c_uint64, C.c_uint32, P]
@@ -172,6 +175,20 @@ def render_batched_mt(b, lo, med, hi, view: View, culling=1, nthreads=8, fb=None
 def resolve_index(fb, rgba, bg):
     out = np.zeros(fb.shape[0], np.uint32)
     load().orc_resolve_index(_p(fb), fb.shape[0], _p(np.ascontiguousarray(rgba, np.uint32)), bg, _p(out))
+    return out
+
+
+def edl_exp(x) -> float:
+    """The oracle's e^x (x <= 0) of the eye-dome lighting shade."""
+    return load().orc_edl_exp(float(x))
+
+
+def resolve_edl(fb, width, height, rgba, bg, strength):
+    """Index-payload resolve with eye-dome lighting (compute_loop_las/resolve.cs:148-191)."""
+    rgba = np.ascontiguousarray(rgba, np.uint32)
+    out = np.zeros(width * height, np.uint32)
+    load().orc_resolve_edl(_p(np.ascontiguousarray(fb, np.uint64)), width, height, _p(rgba), rgba.shape[0], bg,
+                           strength, _p(out))
     return out
 
 

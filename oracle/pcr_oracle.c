@@ -647,6 +647,70 @@ void orc_render_batched(const pcr_batch* b, uint32_t nb, const uint32_t* lo, con
 void orc_resolve_index(const uint64_t* fb, uint64_t npix, const uint32_t* rgba, uint32_t bg, uint32_t* out) {
     for (uint64_t i = 0; i < npix; i++) out[i] = fb[i] == PCR_FB_CLEAR ? bg : rgba[(uint32_t)fb[i]];
 }
+/* e^x for x <= 0, for the eye-dome lighting shade. It is not libm's expf:
+ * the device must produce the same bits, so both sides run this sequence.
+ * Method: Cody-Waite range reduction x = k ln2 + r, with ln2 split into a
+ * high part of 17 significant bits (k * ln2_hi is exact for |k| < 128) and a
+ * low part; e^r from its Taylor series to r^7 in Horner form (|r| <= ln2/2,
+ * truncation error below 2^-27 relative); then scaled by 2^k with ldexp.
+ * Every step is one IEEE operation or an explicit fma. x <= -86 returns 0:
+ * e^-86 < 2^-124, and an 8-bit channel times it truncates to 0 like the
+ * true value. */
+float orc_edl_exp(float x) {
+    if (!(x > -86.0f)) return 0.0f;
+    const float k = rintf(x * 1.44269504f);
+    float r = fmaf(-k, 0.693145751953125f, x);
+    r = fmaf(-k, 1.42860677e-06f, r);
+    float p = 1.98412698e-04f;            /* 1/7! */
+    p = fmaf(p, r, 1.38888889e-03f);      /* 1/6! */
+    p = fmaf(p, r, 8.33333333e-03f);      /* 1/5! */
+    p = fmaf(p, r, 4.16666667e-02f);      /* 1/4! */
+    p = fmaf(p, r, 1.66666667e-01f);      /* 1/3! */
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    return ldexpf(p, (int)k);
+}
+
+/* getDepthAt(pixel, window 0) of compute_loop_las/resolve.cs:43-64: the
+ * pixel's depth if it is in (0, 1e8), else 1e8. A neighbour outside the
+ * image counts as background (the shader reads past the row or buffer). */
+static float edl_depth(const uint64_t* fb, int w, int h, int x, int y) {
+    if (x < 0 || y < 0 || x >= w || y >= h) return 100000000.0f;
+    const float d = bitsf((uint32_t)(fb[(uint64_t)y * w + x] >> 32));
+    return (d > 0.0f && d < 100000000.0f) ? d : 100000000.0f;
+}
+
+/* compute_loop_las/resolve.cs:100-191 with its EDL block switched on: the
+ * colour of the pixel's point, shaded by exp(-(sum/9) * 300 * strength),
+ * where sum adds max(0, depth - neighbour depth) over the 3x3 window in the
+ * shader's ox-major order; channels truncate to integers and alpha is 0.
+ * Background pixels (clear key, or an index outside the cloud) get bg. */
+void orc_resolve_edl(const uint64_t* fb, int w, int h, const uint32_t* rgba, uint64_t npoints, uint32_t bg,
+                     float strength, uint32_t* out) {
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const uint64_t i = (uint64_t)y * w + x;
+            const uint64_t key = fb[i];
+            const uint32_t id = (uint32_t)key;
+            if (key == PCR_FB_CLEAR || id >= npoints) { out[i] = bg; continue; }
+            const uint32_t color = rgba[id];
+            const float depth = bitsf((uint32_t)(key >> 32));
+            float sum = 0.0f;
+            for (int ox = -1; ox <= 1; ox++)
+                for (int oy = -1; oy <= 1; oy++) {
+                    const float dd = depth - edl_depth(fb, w, h, x + ox, y + oy);
+                    sum += dd > 0.0f ? dd : 0.0f;
+                }
+            const float response = sum / 9.0f;
+            const float shade = orc_edl_exp(-response * 300.0f * strength);
+            const uint32_t r = (uint32_t)((float)(color & 255u) * shade);
+            const uint32_t g = (uint32_t)((float)((color >> 8) & 255u) * shade);
+            const uint32_t b = (uint32_t)((float)((color >> 16) & 255u) * shade);
+            out[i] = r | (g << 8) | (b << 16);
+        }
+}
+
 /* compute_2021_earlyz/resolve.cs:41-78: colour carried in the key */
 void orc_resolve_color(const uint64_t* fb, uint64_t npix, uint32_t bg, uint32_t* out) {
     for (uint64_t i = 0; i < npix; i++) out[i] = fb[i] == PCR_FB_CLEAR ? bg : (uint32_t)fb[i];

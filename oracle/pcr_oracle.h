@@ -86,6 +86,10 @@ void orc_render_batched(const pcr_batch* b, uint32_t nb, const uint32_t* lo, con
                         uint64_t* fb, uint8_t* levels_out);
 void orc_resolve_index(const uint64_t* fb, uint64_t npix, const uint32_t* rgba, uint32_t bg, uint32_t* out);
 void orc_resolve_color(const uint64_t* fb, uint64_t npix, uint32_t bg, uint32_t* out);
+/* eye-dome lighting resolve (compute_loop_las/resolve.cs:148-191, switched on) */
+float orc_edl_exp(float x);
+void orc_resolve_edl(const uint64_t* fb, int w, int h, const uint32_t* rgba, uint64_t npoints, uint32_t bg,
+                     float strength, uint32_t* out);
 
 /* hqs: pass 1 depth, pass 2 accumulate, resolve.  src16 != 0 selects the
  * 16 B/pt point source (xyz/rgba) instead of the batched one. */

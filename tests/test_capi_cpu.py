@@ -38,7 +38,7 @@ def test_python_signatures_cover_header():
 def test_struct_sizes_match_header():
     assert C.sizeof(capi.Batch) == 64
     assert C.sizeof(capi.View) == 4 * 64 + 16
-    assert C.sizeof(capi.RenderParams) == 48
+    assert C.sizeof(capi.RenderParams) == 52
 
 
 def test_version_and_device_count():
