@@ -663,9 +663,9 @@ struct HqsTile {
 // are the rectangle and the band count (every thread derives them from the
 // same header and view). A rectangle of at most the tile's width is cut
 // into bands of whole rows that each fit the tile; up to kHqsMaxBands bands
-// re-read and re-project the batch's points once per band, which costs far
-// less than the scattered global atomics of the direct path, and each
-// touched pixel is flushed once, in coalesced rows.
+// re-read and re-project the batch's points once per band, which for a dense
+// batch costs far less than the scattered global atomics of the direct path,
+// and each touched pixel is flushed once, in coalesced rows.
 constexpr int kHqsMaxBands = 8;
 
 template <int BLOCK, int TILE_PIX, int PASS, bool NT>
@@ -685,7 +685,10 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
     const bool fits = on_screen && tw <= cap && h.num_points <= kHqsTileMaxPoints;
     const int bh = fits ? min(th, cap / tw) : 1;    // rows per band
     const int nbands = fits ? (th + bh - 1) / bh : 0;
-    if (!fits || nbands > kHqsMaxBands) {
+    // bands pay when the batch is dense (a point or more per rectangle pixel,
+    // as near batches of a dense cloud are); a sparse one, such as a near LOD
+    // node, costs more to re-project per band than its few atomics do
+    if (!fits || nbands > kHqsMaxBands || (nbands > 1 && (uint64_t)h.num_points < (uint64_t)tw * th)) {
         const TileBand none{0, 0, 0, 0, 0, 0, true};
         hqs_tile_dispatch<BLOCK, PASS, false, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, nullptr, nullptr,
                                                   none);
