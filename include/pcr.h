@@ -121,9 +121,11 @@ typedef struct pcr_render_params {
                                    same result. For hqs it also merges the colour
                                    sums, which is what makes dense views fast    */
     uint32_t batch_begin;       /* draw batches [batch_begin, batch_end) only;     */
-    uint32_t batch_end;         /* 0, 0 = all. One LAS file of a multi-file cloud
-                                   is drawn with its own view (compute_loop_las.h:
-                                   141-199 gives each file its own transform)     */
+    uint32_t batch_end;         /* 0, 0 = all, so an empty range cannot start at
+                                   batch 0: leave such a draw out. One LAS file of
+                                   a multi-file cloud is drawn with its own view
+                                   (compute_loop_las.h:141-199 gives each file its
+                                   own transform)                                 */
     int32_t  resolve_window;    /* PCR_MODE_NODES_HQS: neighbourhood radius of the
                                    resolve, 0..4 (0 in nodes_hqs/resolve.cs:98)    */
     float    edl_strength;      /* LAS/LAS2/NODES resolve: eye-dome lighting, the

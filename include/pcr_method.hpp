@@ -81,7 +81,9 @@ inline std::vector<char> read_file(const std::string& path) {
 }
 
 // LasLoaderSparse's job (LasLoaderSparse.cpp:37-212, 495-526): reads the
-// files, encodes them into one cloud on the device, one part per file.
+// files, encodes them into one cloud on the device, one part per file that
+// has points (an empty file owns no batches, and its empty range would read
+// as "all batches" when it starts at batch 0).
 struct LasFiles {
     pcr_cloud* cloud = nullptr;
     std::vector<CloudPart> parts;
@@ -95,6 +97,7 @@ struct LasFiles {
         for (size_t k = 0; k < files.size(); k++) {
             pcr_las_info info;
             check(pcr_las_parse_header(files[k].bytes, files[k].size, &info));
+            if (info.num_points == 0) continue;
             CloudPart part;
             part.batch_begin = first[k];
             part.batch_end = first[k + 1];

@@ -94,7 +94,11 @@ static int run_files(int argc, char** argv) {
         camera(las.parts[0].origin);
         ComputeLoopLas2 m(ctx, las.cloud, las.parts);
         const long drawn = compare_parts(ctx, r, &m, las.cloud, las.parts);
-        std::printf("%s las_files %zu %ld\n", drawn > 1000 ? "OK" : "FAIL", las.parts.size(), drawn);
+        uint64_t hash = 1469598103934665603ull;   // FNV-1a over the image
+        for (uint32_t px : v.image)
+            for (int b = 0; b < 4; b++) hash = (hash ^ ((px >> (8 * b)) & 255u)) * 1099511628211ull;
+        std::printf("%s las_files %zu %ld %016llx\n", drawn > 1000 ? "OK" : "FAIL", las.parts.size(), drawn,
+                    (unsigned long long)hash);
         failures += drawn <= 1000;
     } else if (std::string(argv[1]) == "--potree" && argc == 4) {
         const std::vector<char> raw = read_file(argv[3]);

@@ -35,6 +35,13 @@ def test_cpp_las_files_through_loop_las_prefetch(tmp_path):
     out = subprocess.run([EXE, "--las", *paths], capture_output=True, text=True, timeout=240)
     print(out.stdout, out.stderr)
     assert out.returncode == 0 and "OK las_files 3" in out.stdout, out.stdout + out.stderr
+    # a file without points, in front: no part, and the same image
+    empty = tmp_path / "empty.las"
+    empty.write_bytes(make_las(0, point_format=2, version=(1, 2), seed=9)[0])
+    out2 = subprocess.run([EXE, "--las", str(empty), *paths], capture_output=True, text=True, timeout=240)
+    print(out2.stdout, out2.stderr)
+    assert out2.returncode == 0 and "OK las_files 3" in out2.stdout, out2.stdout + out2.stderr
+    assert out.stdout.split()[-1] == out2.stdout.split()[-1]
 
 
 def test_cpp_potree_through_loop_nodes(tmp_path):

@@ -35,6 +35,32 @@ def test_upload_validation(ctx, small):
     expect_invalid(lambda: ctx.upload_batched(bad, lo, med, hi, rgba), "parent")
 
 
+def test_empty_and_oversized_inputs(ctx, small):
+    """Clouds need points, framebuffers a size within PCR_MAX_DIM, and an
+    empty batch range away from batch 0 draws nothing."""
+    xyz, rgba, b, lo, med, hi = small
+    z = np.zeros(0, np.uint32)
+    expect_invalid(lambda: ctx.upload_batched(b[:0], z, z, z, z), "bad sizes")
+    expect_invalid(lambda: ctx.upload_points16(np.zeros((0, 3), np.float32), z), "no points")
+    from compute_rasterizer_amd.datasets import make_las
+    expect_invalid(lambda: ctx.from_las([make_las(0, seed=2)[0]]), "0 points")
+    for w, h in ((32769, 8), (8, 32769), (0, 8), (8, -1)):
+        expect_invalid(lambda: ctx.framebuffer(w, h), "bad size")
+    cloud = ctx.upload_batched(b, lo, med, hi, rgba)
+    v = pcr.look_at((1000, -200, 400), (1000, 1000, 50), up=(0, 0, 1), near=1, far=5000, width=320, height=200)
+    fb = ctx.framebuffer(320, 200)
+    for mode in (pcr.MODE_LAS, pcr.MODE_LAS2):
+        ctx.render(cloud, v, pcr.params(mode, batch_begin=3, batch_end=3), fb)
+        assert (fb.keys() == pcr.FB_CLEAR).all()
+    hfb = ctx.framebuffer(320, 200, pcr.FB_HQS)
+    ctx.render(cloud, v, pcr.params(pcr.MODE_HQS, batch_begin=5, batch_end=5), hfb)
+    d, a = hfb.hqs()
+    assert (d == 0xFFFFFFFF).all() and (a == 0).all()
+    fb.close()
+    hfb.close()
+    cloud.close()
+
+
 def test_render_validation(ctx, small):
     xyz, rgba, b, lo, med, hi = small
     cloud = ctx.upload_batched(b, lo, med, hi, rgba)
