@@ -234,18 +234,29 @@ __device__ __forceinline__ bool project_xy(const Proj& p, float x, float y, floa
 // eye or the rectangle is empty. It only routes points to a tile: a point
 // whose pixel falls outside it takes the direct path, so the result never
 // depends on this bound being conservative.
+//
+// Called by every lane of a wave with the same arguments: lane & 7 projects
+// one corner, and the 8-lane groups reduce with xor shuffles, so a wave
+// spends one corner's arithmetic instead of eight (the rectangle is computed
+// once per batch by every wave of the workgroup).
 __device__ inline bool screen_rect(const float* __restrict__ t, const float mn[3], const float mx[3], int w, int h,
                                    int& x0, int& y0, int& x1, int& y1) {
-    float sx0 = 3.0e38f, sx1 = -3.0e38f, sy0 = 3.0e38f, sy1 = -3.0e38f;
+    const int c = (int)(__lane_id() & 7u);
+    const float x = (c & 1) ? mx[0] : mn[0], y = (c & 2) ? mx[1] : mn[1], z = (c & 4) ? mx[2] : mn[2];
+    const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
+    const float py = fma_(t[9], z, fma_(t[5], y, fma_(t[1], x, t[13])));
+    const float pw = fma_(t[11], z, fma_(t[7], y, fma_(t[3], x, t[15])));
+    // every 8-lane group holds the same 8 corners, so any lane not in front
+    // of the eye means a corner that is not
+    if (__ballot(!(pw > 0.0f)) != 0ull) return false;
+    const float sx = fma_(px / pw, 0.5f, 0.5f) * (float)w, sy = fma_(py / pw, 0.5f, 0.5f) * (float)h;
+    float sx0 = sx, sx1 = sx, sy0 = sy, sy1 = sy;
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const float x = (c & 1) ? mx[0] : mn[0], y = (c & 2) ? mx[1] : mn[1], z = (c & 4) ? mx[2] : mn[2];
-        const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
-        const float py = fma_(t[9], z, fma_(t[5], y, fma_(t[1], x, t[13])));
-        const float pw = fma_(t[11], z, fma_(t[7], y, fma_(t[3], x, t[15])));
-        if (!(pw > 0.0f)) return false;
-        const float sx = fma_(px / pw, 0.5f, 0.5f) * (float)w, sy = fma_(py / pw, 0.5f, 0.5f) * (float)h;
-        sx0 = fminf(sx0, sx); sx1 = fmaxf(sx1, sx); sy0 = fminf(sy0, sy); sy1 = fmaxf(sy1, sy);
+    for (int m = 1; m < 8; m <<= 1) {
+        sx0 = fminf(sx0, __shfl_xor(sx0, m));
+        sx1 = fmaxf(sx1, __shfl_xor(sx1, m));
+        sy0 = fminf(sy0, __shfl_xor(sy0, m));
+        sy1 = fmaxf(sy1, __shfl_xor(sy1, m));
     }
     sx0 = fmaxf(sx0, -2.0f); sy0 = fmaxf(sy0, -2.0f);
     sx1 = fminf(sx1, (float)w + 2.0f); sy1 = fminf(sy1, (float)h + 2.0f);
