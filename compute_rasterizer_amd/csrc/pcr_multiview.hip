@@ -570,16 +570,23 @@ int pcr_multiview_destroy(pcr_multiview* mv) {
 }
 
 // Sweeps the untiled kernel over workgroup size {64,128,256,512} x points per
-// thread {40,80,160} and the LDS-tiled kernel over {128,256} x tile
-// {1024,2048,4096} px for the given views; keeps the fastest configuration
-// (median of `iters` graph launches, HIP events).
+// thread {40,80,160} (hqs modes: one untiled candidate) and the LDS-tiled
+// kernel over {128,256} x tile {1024,2048,4096} px for the given views; keeps
+// the fastest configuration (median of `iters` graph launches, HIP events).
 int pcr_autotune(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nviews, pcr_fb* const* fbs,
                  const pcr_render_params* params, int iters, pcr_tune* best, float* best_ms) {
     PCR_CHECK(ctx && cloud && views && fbs && params && best && best_ms, "autotune: null argument");
     PCR_CHECK(iters >= 1 && iters <= 1000, "autotune: iters %d out of range", iters);
     std::vector<pcr_tune> space;
-    for (int bsz : {64, 128, 256, 512})
-        for (int ppt : {40, 80, 160}) space.push_back(pcr_tune{bsz, ppt, 0, 0});
+    const bool hqs = params->mode == PCR_MODE_HQS || params->mode == PCR_MODE_NODES_HQS;
+    if (hqs) {
+        // untiled hqs adds every visible point's colour with global atomics and
+        // runs about ten times slower than the tiled kernel: one candidate
+        space.push_back(pcr_tune{256, 40, 0, 0});
+    } else {
+        for (int bsz : {64, 128, 256, 512})
+            for (int ppt : {40, 80, 160}) space.push_back(pcr_tune{bsz, ppt, 0, 0});
+    }
     for (int bsz : {128, 256})
         for (int tile : {1024, 2048, 4096}) space.push_back(pcr_tune{bsz, 40, 0, tile});
     hipEvent_t e0, e1;

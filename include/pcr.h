@@ -408,8 +408,9 @@ int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, 
 int pcr_multiview_set_views(pcr_multiview* mv, const pcr_view* views, int nviews);
 int pcr_multiview_launch(pcr_multiview* mv);
 int pcr_multiview_destroy(pcr_multiview* mv);
-/* Sweep block size / points per thread for this cloud and views; writes the
- * fastest configuration and its time. */
+/* Sweep block size / points per thread (untiled) and block size / tile size
+ * (LDS-tiled) for this cloud, views and mode; writes the fastest
+ * configuration and its time. hqs modes try one untiled configuration. */
 int pcr_autotune(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nviews,
                  pcr_fb* const* fbs, const pcr_render_params* params, int iters,
                  pcr_tune* best, float* best_ms);

@@ -186,3 +186,50 @@ def test_server_pinned_results(ctx, city):
         srv.release(ts[0])
     assert srv.stats().fb_bytes_in_use == 0
     srv.close()
+
+
+def test_server_concurrent_clients(ctx, city, nodes):
+    """Four host threads submit and collect at once (the server is the one
+    thread-safe object of the library); every image equals the one the same
+    request gets alone, and the budget ends empty."""
+    import threading
+    srv = pcr.Server(ctx, budget_bytes=12 * 1280 * 720 * 6, max_batch=8, num_streams=3)
+    c0, c1 = srv.add_cloud(city), srv.add_cloud(nodes)
+    per_thread = []
+    for t in range(4):
+        vs = views_for(SIZES[:3] * 4, seed=40 + t)
+        per_thread.append([((c0, pcr.MODE_LAS2) if k % 2 == 0 else (c1, pcr.MODE_NODES)) + (v,)
+                           for k, v in enumerate(vs)])
+    alone = {}
+    for t, reqs in enumerate(per_thread):
+        for k, (cid, mode, v) in enumerate(reqs):
+            alone[(t, k)] = srv.wait(srv.submit(cid, v, mode), v.width * v.height)[0]
+    got, errors = {}, []
+
+    def client(t):
+        try:
+            reqs = per_thread[t]
+            tickets = [srv.submit(cid, v, mode) for cid, mode, v in reqs]
+            for k, (tk, (_, _, v)) in enumerate(zip(tickets, reqs)):
+                if k % 3 == 2:   # collect some results zero-copy
+                    img, _ = srv.wait_pinned(tk, v.width * v.height)
+                    got[(t, k)] = img.copy()
+                    srv.release(tk)
+                else:
+                    got[(t, k)] = srv.wait(tk, v.width * v.height)[0]
+        except Exception as e:   # reported below, in the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=client, args=(t,)) for t in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=100)
+    assert not any(th.is_alive() for th in threads)
+    assert not errors, errors
+    assert len(got) == len(alone)
+    for key, img in alone.items():
+        assert np.array_equal(got[key], img), key
+    st = srv.stats()
+    assert st.completed == 2 * len(alone) and st.fb_bytes_in_use == 0
+    srv.close()
