@@ -50,15 +50,24 @@ __device__ __forceinline__ void frame_reset(FrameState* st) {
     for (int i = 0; i < 5; i++) st->level_count[i] = 0;
     st->points_drawn = 0; st->plane_bytes = 0;
 }
+// 16-byte stores (p is hipMalloc-aligned); the last element of an odd-length
+// 64-bit fill, or the last 1-3 of a 32-bit fill, are stored singly.
 __global__ void k_fill64(unsigned long long* __restrict__ p, uint64_t n, unsigned long long v, FrameState* st) {
     if (st && blockIdx.x == 0 && threadIdx.x == 0) frame_reset(st);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+    u64x2* p2 = reinterpret_cast<u64x2*>(p);
+    for (uint64_t i = t; i < n / 2; i += stride) p2[i] = u64x2{v, v};
+    if (t == 0 && (n & 1)) p[n - 1] = v;
 }
 __global__ void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v, FrameState* st) {
     if (st && blockIdx.x == 0 && threadIdx.x == 0) frame_reset(st);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    u32x4* p4 = reinterpret_cast<u32x4*>(p);
+    for (uint64_t i = t; i < n / 4; i += stride) p4[i] = u32x4{v, v, v, v};
+    if (t < (n & 3)) p[(n & ~3ull) + t] = v;
 }
 __global__ void k_frame_reset(FrameState* st) {
     if (threadIdx.x == 0) frame_reset(st);
@@ -72,12 +81,12 @@ static int grid_for(uint64_t n, int block) {
 }
 
 int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v, FrameState* st) {
-    k_fill64<<<grid_for(n, 256), 256, 0, s>>>(p, n, v, st);
+    k_fill64<<<grid_for(n / 2 + 1, 256), 256, 0, s>>>(p, n, v, st);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
 }
 int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v, FrameState* st) {
-    k_fill32<<<grid_for(n, 256), 256, 0, s>>>(p, n, v, st);
+    k_fill32<<<grid_for(n / 4 + 1, 256), 256, 0, s>>>(p, n, v, st);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
 }
