@@ -380,6 +380,16 @@ __device__ __forceinline__ float edl_depth(const unsigned long long* __restrict_
     return (d > 0.0f && d < 100000000.0f) ? d : 100000000.0f;
 }
 
+// Resolve kernels walk the image in 16x16-pixel tiles, one 8x8 quadrant per
+// wave of a 256-thread workgroup: a wave's colour gathers then come from a
+// few neighbouring batches, whose colour-plane lines it shares, rather than
+// from a whole row of them. Sets (x, y) of this thread's pixel in `tile`.
+__device__ __forceinline__ void tile_pixel(uint32_t tile, uint32_t tiles_x, uint32_t& x, uint32_t& y) {
+    const uint32_t lane = threadIdx.x & 63u, wv = (threadIdx.x >> 6) & 3u;
+    x = (tile % tiles_x) * 16u + (wv & 1u) * 8u + (lane & 7u);
+    y = (tile / tiles_x) * 16u + (wv >> 1) * 8u + (lane >> 3);
+}
+
 // The colour of the point in pixel (x, y) for an index-payload framebuffer
 // (compute_loop_las/resolve.cs:100-146): bg for a clear key or an index
 // outside the cloud. With strength > 0, the EDL block the shader carries

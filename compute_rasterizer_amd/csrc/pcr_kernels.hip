@@ -1044,22 +1044,24 @@ int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pc
 // Index payload: colour gathered from the colour plane (resolve.cs:117-146).
 // An index outside the cloud (a framebuffer filled by another cloud or mode)
 // resolves to the background instead of reading out of bounds. strength > 0
-// adds the eye-dome lighting shade (resolve_index_pixel).
-__global__ void k_resolve_index(const unsigned long long* __restrict__ fb, int w, int h,
-                                const uint32_t* __restrict__ rgba, uint64_t npoints, uint32_t bg, float strength,
-                                uint32_t* __restrict__ out) {
-    const uint64_t npix = (uint64_t)w * h;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    if (strength > 0.0f) {   // npix <= 2^30, so 32-bit division finds the pixel's row
-        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride)
-            out[i] = resolve_index_pixel(fb, w, h, (int)((uint32_t)i % (uint32_t)w), (int)((uint32_t)i / (uint32_t)w),
-                                         rgba, npoints, bg, strength);
-        return;
-    }
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
-        const unsigned long long k = fb[i];
-        const uint32_t id = (uint32_t)k;
-        out[i] = (k == PCR_FB_CLEAR || id >= npoints) ? bg : rgba[id];
+// adds the eye-dome lighting shade (resolve_index_pixel). 256 threads per
+// workgroup, one 16x16 tile per iteration (tile_pixel).
+__global__ __launch_bounds__(256) void k_resolve_index(const unsigned long long* __restrict__ fb, int w, int h,
+                                                       const uint32_t* __restrict__ rgba, uint64_t npoints,
+                                                       uint32_t bg, float strength, uint32_t* __restrict__ out) {
+    const uint32_t tiles_x = ((uint32_t)w + 15u) >> 4, tiles = tiles_x * (((uint32_t)h + 15u) >> 4);
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        uint32_t x, y;
+        tile_pixel(tile, tiles_x, x, y);
+        if (x >= (uint32_t)w || y >= (uint32_t)h) continue;
+        const uint64_t i = (uint64_t)y * (uint32_t)w + x;
+        if (strength > 0.0f) {
+            out[i] = resolve_index_pixel(fb, w, h, (int)x, (int)y, rgba, npoints, bg, strength);
+        } else {
+            const unsigned long long k = fb[i];
+            const uint32_t id = (uint32_t)k;
+            out[i] = (k == PCR_FB_CLEAR || id >= npoints) ? bg : rgba[id];
+        }
     }
 }
 // Colour payload: the key's low 32 bits are the RGBA (2021 resolve.cs:45-57).

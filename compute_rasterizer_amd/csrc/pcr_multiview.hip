@@ -189,25 +189,29 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster(const pcr_batch* __restrict
     }
 }
 
-__global__ void k_mv_resolve(const unsigned long long* const* __restrict__ fbs, uint32_t* const* __restrict__ out,
-                             const pcr_view* __restrict__ views, const uint32_t* __restrict__ rgba, uint64_t npoints,
-                             uint32_t bg, float edl) {
+// One view per blockIdx.y, its pixels in 16x16 tiles as k_resolve_index.
+__global__ __launch_bounds__(256) void k_mv_resolve(const unsigned long long* const* __restrict__ fbs,
+                                                    uint32_t* const* __restrict__ out,
+                                                    const pcr_view* __restrict__ views,
+                                                    const uint32_t* __restrict__ rgba, uint64_t npoints, uint32_t bg,
+                                                    float edl) {
     const uint32_t v = blockIdx.y;
     const int w = views[v].width, h = views[v].height;
-    const uint64_t npix = (uint64_t)w * h;
     const unsigned long long* fb = fbs[v];
     uint32_t* o = out[v];
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    if (edl > 0.0f) {   // npix <= 2^30, so 32-bit division finds the pixel's row
-        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride)
-            o[i] = resolve_index_pixel(fb, w, h, (int)((uint32_t)i % (uint32_t)w), (int)((uint32_t)i / (uint32_t)w),
-                                       rgba, npoints, bg, edl);
-        return;
-    }
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
-        const unsigned long long k = fb[i];
-        const uint32_t id = (uint32_t)k;
-        o[i] = (k == PCR_FB_CLEAR || id >= npoints) ? bg : rgba[id];
+    const uint32_t tiles_x = ((uint32_t)w + 15u) >> 4, tiles = tiles_x * (((uint32_t)h + 15u) >> 4);
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        uint32_t x, y;
+        tile_pixel(tile, tiles_x, x, y);
+        if (x >= (uint32_t)w || y >= (uint32_t)h) continue;
+        const uint64_t i = (uint64_t)y * (uint32_t)w + x;
+        if (edl > 0.0f) {
+            o[i] = resolve_index_pixel(fb, w, h, (int)x, (int)y, rgba, npoints, bg, edl);
+        } else {
+            const unsigned long long k = fb[i];
+            const uint32_t id = (uint32_t)k;
+            o[i] = (k == PCR_FB_CLEAR || id >= npoints) ? bg : rgba[id];
+        }
     }
 }
 
