@@ -1080,13 +1080,18 @@ __global__ void k_resolve_hqs(const u32x4* __restrict__ acc, uint64_t npix, uint
         out[i] = hqs_average(acc[i], bg);
 }
 
-// compute_loop_nodes_hqs/resolve.cs:92-158 (hqs_window_pixel).
-__global__ void k_resolve_hqs_window(const uint32_t* __restrict__ depth, const u32x4* __restrict__ acc, int w, int h,
-                                     int window, uint32_t* __restrict__ out) {
-    const uint64_t npix = (uint64_t)w * h;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride)
-        out[i] = hqs_window_pixel(depth, acc, w, h, (int)(i % (uint64_t)w), (int)(i / (uint64_t)w), window);
+// compute_loop_nodes_hqs/resolve.cs:92-158 (hqs_window_pixel), in 16x16
+// tiles as k_resolve_index, so the neighbourhoods a wave reads overlap.
+__global__ __launch_bounds__(256) void k_resolve_hqs_window(const uint32_t* __restrict__ depth,
+                                                            const u32x4* __restrict__ acc, int w, int h, int window,
+                                                            uint32_t* __restrict__ out) {
+    const uint32_t tiles_x = ((uint32_t)w + 15u) >> 4, tiles = tiles_x * (((uint32_t)h + 15u) >> 4);
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        uint32_t x, y;
+        tile_pixel(tile, tiles_x, x, y);
+        if (x >= (uint32_t)w || y >= (uint32_t)h) continue;
+        out[(uint64_t)y * (uint32_t)w + x] = hqs_window_pixel(depth, acc, w, h, (int)x, (int)y, window);
+    }
 }
 
 int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, float edl, pcr_fb* fb) {

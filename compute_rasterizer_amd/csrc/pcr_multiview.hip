@@ -329,18 +329,29 @@ __global__ __launch_bounds__(BLOCK) void k_mv_hqs(const pcr_batch* __restrict__ 
 
 // window < 0: integer averages with the background (HQS); otherwise the
 // weighted neighbourhood average of NODES_HQS.
-__global__ void k_mv_resolve_hqs(uint32_t* const* __restrict__ depths, uint32_t* const* __restrict__ accs,
-                                 uint32_t* const* __restrict__ out, const pcr_view* __restrict__ views, int window,
-                                 uint32_t bg) {
+__global__ __launch_bounds__(256) void k_mv_resolve_hqs(uint32_t* const* __restrict__ depths,
+                                                        uint32_t* const* __restrict__ accs,
+                                                        uint32_t* const* __restrict__ out,
+                                                        const pcr_view* __restrict__ views, int window, uint32_t bg) {
     const uint32_t v = blockIdx.y;
     const int w = views[v].width, h = views[v].height;
-    const uint64_t npix = (uint64_t)w * h;
     const uint32_t* dp = depths[v];
     const u32x4* ac = reinterpret_cast<const u32x4*>(accs[v]);
     uint32_t* o = out[v];
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (uint64_t)gridDim.x * blockDim.x)
-        o[i] = window < 0 ? hqs_average(ac[i], bg)
-                          : hqs_window_pixel(dp, ac, w, h, (int)(i % (uint64_t)w), (int)(i / (uint64_t)w), window);
+    if (window < 0) {
+        const uint64_t npix = (uint64_t)w * h;
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix;
+             i += (uint64_t)gridDim.x * blockDim.x)
+            o[i] = hqs_average(ac[i], bg);
+        return;
+    }
+    const uint32_t tiles_x = ((uint32_t)w + 15u) >> 4, tiles = tiles_x * (((uint32_t)h + 15u) >> 4);
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {   // as k_resolve_hqs_window
+        uint32_t x, y;
+        tile_pixel(tile, tiles_x, x, y);
+        if (x >= (uint32_t)w || y >= (uint32_t)h) continue;
+        o[(uint64_t)y * (uint32_t)w + x] = hqs_window_pixel(dp, ac, w, h, (int)x, (int)y, window);
+    }
 }
 
 int mv_alloc(MVBuffers& B, int nviews, uint64_t nb) {
