@@ -426,6 +426,8 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
     const bool pts16 = mode == PCR_MODE_PLAIN16 || mode == PCR_MODE_HQS16;
     const bool lod = mode == PCR_MODE_NODES || mode == PCR_MODE_NODES_HQS;
     PCR_CHECK(mode >= PCR_MODE_PLAIN16 && mode <= PCR_MODE_NODES_HQS, "render: unknown mode %d", mode);
+    PCR_CHECK(params->time_stages >= 0 && params->time_stages <= 2, "render: time_stages %d not in 0..2",
+              params->time_stages);
     PCR_CHECK(hqs == (fb->kind == PCR_FB_HQS), "render: mode %d needs a %s framebuffer", mode,
               hqs ? "HQS" : "KEY64");
     PCR_CHECK(pts16 ? cloud->d_points16 != nullptr : cloud->d_batches != nullptr,
@@ -443,22 +445,30 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
     PCR_CHECK(block == 0 || block == 64 || block == 128 || block == 256 || block == 512,
               "render: block_size %d not in {64,128,256,512}", block);
     hipStream_t s = ctx->stream;
-    ctx->timed = params->time_stages != 0;
-    hipEvent_t* ev = ctx->ring[ctx->frames_timed % PCR_TIMING_RING];
-    if (ctx->timed) { ctx->frames_timed++; PCR_HIP(hipEventRecord(ev[0], s)); }
+    // 1: events around every stage; 2: around the raster stage only (two
+    // events a frame instead of six, each of which costs the stream a few µs)
+    ctx->timing = params->time_stages;
+    const bool all = ctx->timing == 1;
+    const uint64_t slot = ctx->frames_timed % PCR_TIMING_RING;
+    hipEvent_t* ev = ctx->ring[slot];
+    if (ctx->timing) {
+        ctx->ring_level[slot] = (uint8_t)ctx->timing;
+        ctx->frames_timed++;
+    }
+    if (all) PCR_HIP(hipEventRecord(ev[0], s));
     int rc = PCR_OK;
     if (params->clear_before) {
         if ((rc = clear_fb(s, fb, ctx->d_state))) return rc;
     } else if ((rc = launch_frame_reset(s, ctx->d_state))) {
         return rc;
     }
-    if (ctx->timed) PCR_HIP(hipEventRecord(ev[1], s));
+    if (all) PCR_HIP(hipEventRecord(ev[1], s));
     if (!pts16) {
         rc = launch_select(s, cloud, *view, *params, ctx->d_state, lod ? 80.0f : 100.0f, lod);
         if (rc) return rc;
     }
     ctx->last_nb = pts16 ? 0 : cloud->num_batches;
-    if (ctx->timed) PCR_HIP(hipEventRecord(ev[2], s));
+    if (ctx->timing) PCR_HIP(hipEventRecord(ev[2], s));
     switch (mode) {
         case PCR_MODE_PLAIN16: rc = launch_raster_points16(s, cloud, *view, fb->d_keys); break;
         case PCR_MODE_LAS: rc = launch_raster_batched(s, cloud, *view, 1, block ? block : 128, ctx->d_state, fb->d_keys); break;
@@ -475,7 +485,7 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
         case PCR_MODE_HQS16: rc = launch_hqs_points16(s, cloud, *view, fb); break;
     }
     if (rc) return rc;
-    if (ctx->timed) PCR_HIP(hipEventRecord(ev[3], s));
+    if (ctx->timing) PCR_HIP(hipEventRecord(ev[3], s));
     return PCR_OK;
 }
 
@@ -493,11 +503,11 @@ int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params,
               (double)params->edl_strength);
     PCR_CHECK(hqs || mode == PCR_MODE_PLAIN16 || cloud->d_rgba, "resolve: cloud has no colour plane");
     hipEvent_t* ev = ctx->ring[(ctx->frames_timed + PCR_TIMING_RING - 1) % PCR_TIMING_RING];
-    if (ctx->timed) PCR_HIP(hipEventRecord(ev[4], ctx->stream));
+    if (ctx->timing == 1) PCR_HIP(hipEventRecord(ev[4], ctx->stream));
     int rc = launch_resolve(ctx->stream, cloud, mode, params->background, params->resolve_window,
                             params->edl_strength, fb);
     if (rc) return rc;
-    if (ctx->timed) PCR_HIP(hipEventRecord(ev[5], ctx->stream));
+    if (ctx->timing == 1) PCR_HIP(hipEventRecord(ev[5], ctx->stream));
     return PCR_OK;
 }
 
@@ -512,13 +522,13 @@ int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out) {
     for (int i = 0; i < 5; i++) out->batches_per_level[i] = st.level_count[i];
     out->points_drawn = st.points_drawn;
     out->bytes_algorithmic = st.plane_bytes + 64ull * ctx->last_nb;
-    if (ctx->timed && ctx->frames_timed) {
+    if (ctx->timing && ctx->frames_timed) {
         hipEvent_t* ev = ctx->ring[(ctx->frames_timed - 1) % PCR_TIMING_RING];
         float ms = 0;
-        if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) out->ms_clear = ms;
-        if (hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) out->ms_cull = ms;
+        if (ctx->timing == 1 && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) out->ms_clear = ms;
+        if (ctx->timing == 1 && hipEventElapsedTime(&ms, ev[1], ev[2]) == hipSuccess) out->ms_cull = ms;
         if (hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) out->ms_raster = ms;
-        if (hipEventElapsedTime(&ms, ev[4], ev[5]) == hipSuccess) out->ms_resolve = ms;
+        if (ctx->timing == 1 && hipEventElapsedTime(&ms, ev[4], ev[5]) == hipSuccess) out->ms_resolve = ms;
         (void)hipGetLastError();
     }
     ctx->last = *out;
@@ -530,16 +540,31 @@ int pcr_frame_times(pcr_ctx* ctx, int n, float* out5, int* got) {
     PCR_HIP(hipStreamSynchronize(ctx->stream));
     uint64_t avail = ctx->frames_timed < PCR_TIMING_RING ? ctx->frames_timed : PCR_TIMING_RING;
     if ((uint64_t)n < avail) avail = (uint64_t)n;
+    auto elapsed = [](hipEvent_t a, hipEvent_t b) {
+        float ms = -1.0f;
+        if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+            (void)hipGetLastError();
+            ms = -1.0f;
+        }
+        return ms;
+    };
     for (uint64_t k = 0; k < avail; k++) {
-        hipEvent_t* ev = ctx->ring[(ctx->frames_timed - avail + k) % PCR_TIMING_RING];
-        const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {0, 5}};
-        for (int j = 0; j < 5; j++) {
-            float ms = -1.0f;
-            if (hipEventElapsedTime(&ms, ev[pairs[j][0]], ev[pairs[j][1]]) != hipSuccess) {
-                (void)hipGetLastError();
-                ms = -1.0f;
-            }
-            out5[5 * k + j] = ms;
+        const uint64_t f = ctx->frames_timed - avail + k;   // frame number
+        hipEvent_t* ev = ctx->ring[f % PCR_TIMING_RING];
+        float* o = out5 + 5 * k;
+        o[2] = elapsed(ev[2], ev[3]);
+        if (ctx->ring_level[f % PCR_TIMING_RING] == 1) {
+            o[0] = elapsed(ev[0], ev[1]);
+            o[1] = elapsed(ev[1], ev[2]);
+            o[3] = elapsed(ev[4], ev[5]);
+            o[4] = elapsed(ev[0], ev[5]);
+        } else {
+            // raster events only: no stage times, and the frame is the period
+            // from the previous frame's raster end to this one's, when the
+            // ring still holds that frame
+            o[0] = o[1] = o[3] = -1.0f;
+            const bool prev = f > 0 && ctx->frames_timed - f < PCR_TIMING_RING;
+            o[4] = prev ? elapsed(ctx->ring[(f - 1) % PCR_TIMING_RING][3], ev[3]) : -1.0f;
         }
     }
     *got = (int)avail;

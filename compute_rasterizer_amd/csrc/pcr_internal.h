@@ -42,7 +42,8 @@ struct pcr_ctx {
     pcr::FrameState* d_state = nullptr;   // one FrameState per context
     hipEvent_t (*ring)[6] = nullptr;      // stage timers, PCR_TIMING_RING frames x 6 events
     uint64_t frames_timed = 0;            // frames that recorded events (ring head)
-    bool timed = false;                   // the current frame records stage events
+    int timing = 0;                       // the current frame's time_stages (0, 1, 2)
+    uint8_t ring_level[PCR_TIMING_RING] = {};   // time_stages each ring slot was recorded with
     pcr_stats last{};                     // host copy of the last frame's stats
     uint32_t last_nb = 0;
 };

@@ -114,7 +114,9 @@ typedef struct pcr_render_params {
     uint32_t background;        /* RGBA8 written where nothing was drawn           */
     int32_t  block_size;        /* raster workgroup size: 0 = default (256)        */
     int32_t  clear_before;      /* 1: clear the framebuffer before rasterising     */
-    int32_t  time_stages;       /* 1: record HIP events around each stage          */
+    int32_t  time_stages;       /* 1: record HIP events around each stage; 2:
+                                   around the raster stage only (two events a
+                                   frame instead of six; see pcr_frame_times)    */
     int32_t  tile_px;           /* LAS2/NODES/HQS/NODES_HQS: batches whose screen
                                    rectangle fits in this many pixels merge in an
                                    LDS tile first (0 = off, else 1024/2048/4096);
@@ -370,11 +372,14 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view,
  * hqs/resolve.cs:62-129, compute_2021_earlyz/resolve.cs:41-78). */
 int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params, pcr_fb* fb);
 int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out);   /* synchronises */
-/* Stage times of the last n frames rendered with time_stages = 1, oldest
+/* Stage times of the last n frames rendered with time_stages != 0, oldest
  * first, five floats per frame: clear, cull, raster, resolve and the whole
- * frame from clear to the end of resolve (ms). Up to
- * PCR_TIMING_RING frames are kept (GLTimerQueries.cpp:55-153 kept 3 frames of
- * GL timestamps for the same purpose). Synchronises; *got = frames written. */
+ * frame from clear to the end of resolve (ms). A frame rendered with
+ * time_stages = 2 has only the raster time; its clear, cull and resolve read
+ * -1, and its whole-frame time is the period from the previous timed frame's
+ * raster end to its own. Up to PCR_TIMING_RING frames are kept
+ * (GLTimerQueries.cpp:55-153 kept 3 frames of GL timestamps for the same
+ * purpose). Synchronises; *got = frames written. */
 #define PCR_TIMING_RING 512
 int pcr_frame_times(pcr_ctx* ctx, int n, float* out5, int* got);
 /* Device check of the projection's division (no counterpart in the

@@ -116,3 +116,28 @@ def test_repeated_frames_identical(ctx, tile):
         assert np.array_equal(k, first)
     assert (first != pcr.FB_CLEAR).sum() > 10_000
     cloud.close()
+
+
+def test_frame_times_levels(ctx, small):
+    """time_stages 1 records every stage; 2 only the raster, with the frame
+    period taken from raster end to raster end; other values are refused."""
+    xyz, rgba, b, lo, med, hi = small
+    cloud = ctx.upload_batched(b, lo, med, hi, rgba)
+    v = pcr.look_at((1000, -200, 400), (1000, 1000, 50), up=(0, 0, 1), near=1, far=5000, width=320, height=200)
+    fb = ctx.framebuffer(320, 200)
+    expect_invalid(lambda: ctx.render(cloud, v, pcr.params(pcr.MODE_LAS2, time_stages=3), fb), "time_stages")
+    for level in (1, 2):
+        prm = pcr.params(pcr.MODE_LAS2, time_stages=level)
+        for _ in range(4):
+            ctx.render(cloud, v, prm, fb)
+            ctx.resolve(cloud, prm, fb)
+        ft = ctx.frame_times(3)
+        assert ft.shape == (3, 5)
+        assert np.all(ft[:, 2] > 0)
+        assert np.all(ft[:, 4] >= ft[:, 2])
+        if level == 1:
+            assert np.all(ft[:, [0, 1, 3]] > 0)
+        else:
+            assert np.all(ft[:, [0, 1, 3]] == -1)
+    fb.close()
+    cloud.close()
