@@ -73,8 +73,8 @@ int pcr_create(int device, pcr_ctx** out) {
     PCR_HIP(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
     PCR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    PCR_HIP(hipMalloc(&c->d_state, sizeof(FrameState)));
-    PCR_HIP(hipMemset(c->d_state, 0, sizeof(FrameState)));
+    PCR_HIP(hipMalloc(&c->d_state, 2 * sizeof(FrameState)));
+    PCR_HIP(hipMemset(c->d_state, 0, 2 * sizeof(FrameState)));
     c->ring = new hipEvent_t[PCR_TIMING_RING][6];
     for (int f = 0; f < PCR_TIMING_RING; f++)
         for (int k = 0; k < 6; k++) PCR_HIP(hipEventCreate(&c->ring[f][k]));
@@ -457,30 +457,46 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
     }
     if (all) PCR_HIP(hipEventRecord(ev[0], s));
     int rc = PCR_OK;
-    if (params->clear_before) {
-        if ((rc = clear_fb(s, fb, ctx->d_state))) return rc;
-    } else if ((rc = launch_frame_reset(s, ctx->d_state))) {
-        return rc;
-    }
+    FrameState* st = ctx->d_state + ctx->cur;
+    FrameState* st_next = ctx->d_state + (1 - ctx->cur);
+    if (!ctx->state_clean[ctx->cur] && (rc = launch_frame_reset(s, st))) return rc;
+    // batched modes clear the framebuffer inside the select launch; the
+    // 16 B/pt modes have no select and clear with their own fill
+    if (params->clear_before && pts16 && (rc = clear_fb(s, fb, nullptr))) return rc;
     if (all) PCR_HIP(hipEventRecord(ev[1], s));
+    ctx->last_state = ctx->cur;
     if (!pts16) {
-        rc = launch_select(s, cloud, *view, *params, ctx->d_state, lod ? 80.0f : 100.0f, lod);
+        const uint64_t npix = (uint64_t)fb->width * fb->height;
+        FillRegion f0, f1;
+        if (params->clear_before) {
+            if (fb->kind == PCR_FB_KEY64) {
+                f0 = FillRegion{reinterpret_cast<uint32_t*>(fb->d_keys), 2 * npix, 0xFFFFFFFFu};   // PCR_FB_CLEAR
+            } else {
+                f0 = FillRegion{fb->d_depth, npix, PCR_DEPTH_CLEAR};
+                f1 = FillRegion{fb->d_acc, 4 * npix, 0u};
+            }
+        }
+        rc = launch_select(s, cloud, *view, *params, st, st_next, f0, f1, lod ? 80.0f : 100.0f, lod);
         if (rc) return rc;
+        // this frame's state is in use; the select reset the other one
+        ctx->state_clean[ctx->cur] = false;
+        ctx->state_clean[1 - ctx->cur] = true;
+        ctx->cur = 1 - ctx->cur;
     }
     ctx->last_nb = pts16 ? 0 : cloud->num_batches;
     if (ctx->timing) PCR_HIP(hipEventRecord(ev[2], s));
     switch (mode) {
         case PCR_MODE_PLAIN16: rc = launch_raster_points16(s, cloud, *view, fb->d_keys); break;
-        case PCR_MODE_LAS: rc = launch_raster_batched(s, cloud, *view, 1, block ? block : 128, ctx->d_state, fb->d_keys); break;
+        case PCR_MODE_LAS: rc = launch_raster_batched(s, cloud, *view, 1, block ? block : 128, st, fb->d_keys); break;
         case PCR_MODE_LAS2:
         case PCR_MODE_NODES:
-            rc = params->tile_px ? launch_raster_tiled(s, cloud, *view, params->tile_px, block, ctx->d_state, fb->d_keys)
-                                 : launch_raster_batched(s, cloud, *view, 4, block ? block : 256, ctx->d_state, fb->d_keys);
+            rc = params->tile_px ? launch_raster_tiled(s, cloud, *view, params->tile_px, block, st, fb->d_keys)
+                                 : launch_raster_batched(s, cloud, *view, 4, block ? block : 256, st, fb->d_keys);
             break;
         case PCR_MODE_HQS:
         case PCR_MODE_NODES_HQS:
-            rc = params->tile_px ? launch_hqs_tiled(s, cloud, *view, params->tile_px, block, ctx->d_state, fb)
-                                 : launch_hqs_batched(s, cloud, *view, ctx->d_state, fb);
+            rc = params->tile_px ? launch_hqs_tiled(s, cloud, *view, params->tile_px, block, st, fb)
+                                 : launch_hqs_batched(s, cloud, *view, st, fb);
             break;
         case PCR_MODE_HQS16: rc = launch_hqs_points16(s, cloud, *view, fb); break;
     }
@@ -515,7 +531,7 @@ int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out) {
     PCR_CHECK(ctx && out, "stats: null argument");
     PCR_HIP(hipStreamSynchronize(ctx->stream));
     FrameState st;
-    PCR_HIP(hipMemcpy(&st, ctx->d_state, sizeof(st), hipMemcpyDeviceToHost));
+    PCR_HIP(hipMemcpy(&st, ctx->d_state + ctx->last_state, sizeof(st), hipMemcpyDeviceToHost));
     memset(out, 0, sizeof(*out));
     out->batches_total = ctx->last_nb;
     out->batches_culled = st.culled;

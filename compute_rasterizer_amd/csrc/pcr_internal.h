@@ -39,7 +39,13 @@ struct pcr_ctx {
     int device = 0;
     int cu_count = 0;
     hipStream_t stream = nullptr;
-    pcr::FrameState* d_state = nullptr;   // one FrameState per context
+    // Two frame states: a frame uses d_state[cur], and its select launch
+    // resets the other one for the next frame, so no launch of its own has
+    // to reset the state before the select's atomics
+    pcr::FrameState* d_state = nullptr;
+    int cur = 0;                          // the state the next frame uses
+    int last_state = 0;                   // the state the last frame used (pcr_stats_get)
+    bool state_clean[2] = {true, true};   // the state is known to be reset
     hipEvent_t (*ring)[6] = nullptr;      // stage timers, PCR_TIMING_RING frames x 6 events
     uint64_t frames_timed = 0;            // frames that recorded events (ring head)
     int timing = 0;                       // the current frame's time_stages (0, 1, 2)
@@ -87,8 +93,17 @@ namespace pcr {
 int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v, FrameState* st = nullptr);
 int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v, FrameState* st = nullptr);
 int launch_frame_reset(hipStream_t s, FrameState* st);
+// A framebuffer region to fill with one 32-bit word value (n words).
+struct FillRegion {
+    uint32_t* p = nullptr;
+    uint64_t n = 0;
+    uint32_t v = 0;
+};
+// st: the frame's state (reset); st_next: reset here for the next frame;
+// fill0/fill1: framebuffer regions cleared in the same launch.
 int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pcr_render_params& p,
-                  FrameState* st, float first_threshold, bool lod);
+                  FrameState* st, FrameState* st_next, const FillRegion& fill0, const FillRegion& fill1,
+                  float first_threshold, bool lod);
 int launch_raster_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int vec, int block,
                           FrameState* st, unsigned long long* fb);
 int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block,

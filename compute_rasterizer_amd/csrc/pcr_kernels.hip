@@ -103,13 +103,31 @@ int launch_frame_reset(hipStream_t s, FrameState* st) {
 // workgroup's 16 waves in LDS first, so the frame's shared counters see one
 // list atomic and eight statistics atomics per 1024 batches.
 constexpr int kSelectBlock = 1024;
+// The same launch clears the frame's framebuffer (fill0, fill1: 16-byte
+// stores over the whole grid; the raster runs after it on the stream) and
+// resets the other frame state for the next frame, so a frame needs no
+// launch before its select.
+__device__ __forceinline__ void fill_words(const FillRegion& f, uint64_t t, uint64_t stride) {
+    u32x4* p4 = reinterpret_cast<u32x4*>(f.p);
+    for (uint64_t i = t; i < f.n / 4; i += stride) p4[i] = u32x4{f.v, f.v, f.v, f.v};
+    if (t < (f.n & 3)) f.p[(f.n & ~3ull) + t] = f.v;
+}
+
 template <bool LOD>
 __global__ __launch_bounds__(kSelectBlock) void k_select(const pcr_batch* __restrict__ b, uint32_t nb, pcr_view v,
                                                          int culling, int rule, float lod_px, float first,
                                                          uint32_t b0, uint32_t b1,
                                                          uint8_t* __restrict__ levels, uint32_t* __restrict__ list,
-                                                         FrameState* __restrict__ st) {
+                                                         FrameState* __restrict__ st, FrameState* __restrict__ st_next,
+                                                         FillRegion fill0, FillRegion fill1) {
     constexpr int kWaves = kSelectBlock / 64;
+    {
+        const uint64_t t = (uint64_t)blockIdx.x * kSelectBlock + threadIdx.x;
+        const uint64_t stride = (uint64_t)gridDim.x * kSelectBlock;
+        if (t == 0) frame_reset(st_next);
+        fill_words(fill0, t, stride);
+        fill_words(fill1, t, stride);
+    }
     __shared__ float planes[6][4];   // the view's frustum planes, computed once per workgroup
     __shared__ uint32_t s_keep[kWaves];
     __shared__ unsigned long long s_stat[8][kWaves];   // culled, level 0..4, points, bytes
@@ -181,17 +199,23 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(const pcr_batch* __rest
 }
 
 int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pcr_render_params& p,
-                  FrameState* st, float first, bool lod) {
-    const int grid = (int)((c->num_batches + kSelectBlock - 1) / kSelectBlock);
-    if (grid == 0) return PCR_OK;
+                  FrameState* st, FrameState* st_next, const FillRegion& fill0, const FillRegion& fill1,
+                  float first, bool lod) {
+    // enough workgroups for the batches, and for the fill (at most 512)
+    const uint64_t sel = (c->num_batches + kSelectBlock - 1) / kSelectBlock;
+    const uint64_t words = fill0.n > fill1.n ? fill0.n : fill1.n;
+    uint64_t fg = (words / 4 + kSelectBlock - 1) / kSelectBlock;
+    if (fg > 512) fg = 512;
+    const int grid = (int)(sel > fg ? sel : (fg ? fg : 1));
     const uint32_t b0 = p.batch_begin, b1 = p.batch_end ? p.batch_end : c->num_batches;
     if (lod)
         k_select<true><<<grid, kSelectBlock, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling, p.lod_rule,
-                                                     p.lod_px, first, b0, b1, c->d_levels, c->d_list, st);
+                                                     p.lod_px, first, b0, b1, c->d_levels, c->d_list, st, st_next,
+                                                     fill0, fill1);
     else
         k_select<false><<<grid, kSelectBlock, 0, s>>>(c->d_batches, c->num_batches, v, p.frustum_culling,
                                                       p.lod_rule, p.lod_px, first, b0, b1, c->d_levels, c->d_list,
-                                                      st);
+                                                      st, st_next, fill0, fill1);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
 }

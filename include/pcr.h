@@ -374,7 +374,9 @@ int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params,
 int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out);   /* synchronises */
 /* Stage times of the last n frames rendered with time_stages != 0, oldest
  * first, five floats per frame: clear, cull, raster, resolve and the whole
- * frame from clear to the end of resolve (ms). A frame rendered with
+ * frame from clear to the end of resolve (ms). Batched modes clear the
+ * framebuffer inside the cull launch, so their clear time is about 0 and
+ * their cull time includes the clear. A frame rendered with
  * time_stages = 2 has only the raster time; its clear, cull and resolve read
  * -1, and its whole-frame time is the period from the previous timed frame's
  * raster end to its own. Up to PCR_TIMING_RING frames are kept
