@@ -31,10 +31,10 @@ static bool valid_view(const pcr_view* v) {
     return v && v->width > 0 && v->height > 0 && v->width <= PCR_MAX_DIM && v->height <= PCR_MAX_DIM;
 }
 
-static int clear_fb(hipStream_t s, pcr_fb* fb, FrameState* st = nullptr) {
+static int clear_fb(hipStream_t s, pcr_fb* fb) {
     const uint64_t npix = (uint64_t)fb->width * fb->height;
-    if (fb->kind == PCR_FB_KEY64) return launch_fill64(s, fb->d_keys, npix, PCR_FB_CLEAR, st);
-    int rc = launch_fill32(s, fb->d_depth, npix, PCR_DEPTH_CLEAR, st);
+    if (fb->kind == PCR_FB_KEY64) return launch_fill64(s, fb->d_keys, npix, PCR_FB_CLEAR);
+    int rc = launch_fill32(s, fb->d_depth, npix, PCR_DEPTH_CLEAR);
     if (rc) return rc;
     return launch_fill32(s, fb->d_acc, 4 * npix, 0u);
 }
@@ -462,7 +462,7 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
     if (!ctx->state_clean[ctx->cur] && (rc = launch_frame_reset(s, st))) return rc;
     // batched modes clear the framebuffer inside the select launch; the
     // 16 B/pt modes have no select and clear with their own fill
-    if (params->clear_before && pts16 && (rc = clear_fb(s, fb, nullptr))) return rc;
+    if (params->clear_before && pts16 && (rc = clear_fb(s, fb))) return rc;
     if (all) PCR_HIP(hipEventRecord(ev[1], s));
     ctx->last_state = ctx->cur;
     if (!pts16) {

@@ -89,9 +89,8 @@ struct pcr_fb {
 namespace pcr {
 
 // kernels (pcr_kernels.hip)
-// st: also reset that frame state (the clear that opens a frame)
-int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v, FrameState* st = nullptr);
-int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v, FrameState* st = nullptr);
+int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v);
+int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
 int launch_frame_reset(hipStream_t s, FrameState* st);
 // A framebuffer region to fill with one 32-bit word value (n words).
 struct FillRegion {

@@ -43,8 +43,8 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // ------------------------------------------------------------------ fills
-// The frame's counters reset with the clear (one launch fewer per frame):
-// thread 0 of workgroup 0 zeroes them when `st` is given.
+// A frame's counters are zeroed by the select launch of the frame before it
+// (k_select resets the other of the context's two frame states).
 __device__ __forceinline__ void frame_reset(FrameState* st) {
     st->list_count = 0; st->queue_head = 0; st->queue_head2 = 0; st->culled = 0;
     for (int i = 0; i < 5; i++) st->level_count[i] = 0;
@@ -52,8 +52,7 @@ __device__ __forceinline__ void frame_reset(FrameState* st) {
 }
 // 16-byte stores (p is hipMalloc-aligned); the last element of an odd-length
 // 64-bit fill, or the last 1-3 of a 32-bit fill, are stored singly.
-__global__ void k_fill64(unsigned long long* __restrict__ p, uint64_t n, unsigned long long v, FrameState* st) {
-    if (st && blockIdx.x == 0 && threadIdx.x == 0) frame_reset(st);
+__global__ void k_fill64(unsigned long long* __restrict__ p, uint64_t n, unsigned long long v) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
@@ -61,8 +60,7 @@ __global__ void k_fill64(unsigned long long* __restrict__ p, uint64_t n, unsigne
     for (uint64_t i = t; i < n / 2; i += stride) p2[i] = u64x2{v, v};
     if (t == 0 && (n & 1)) p[n - 1] = v;
 }
-__global__ void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v, FrameState* st) {
-    if (st && blockIdx.x == 0 && threadIdx.x == 0) frame_reset(st);
+__global__ void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     u32x4* p4 = reinterpret_cast<u32x4*>(p);
@@ -80,13 +78,13 @@ static int grid_for(uint64_t n, int block) {
     return (int)g;
 }
 
-int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v, FrameState* st) {
-    k_fill64<<<grid_for(n / 2 + 1, 256), 256, 0, s>>>(p, n, v, st);
+int launch_fill64(hipStream_t s, unsigned long long* p, uint64_t n, unsigned long long v) {
+    k_fill64<<<grid_for(n / 2 + 1, 256), 256, 0, s>>>(p, n, v);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
 }
-int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v, FrameState* st) {
-    k_fill32<<<grid_for(n / 4 + 1, 256), 256, 0, s>>>(p, n, v, st);
+int launch_fill32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v) {
+    k_fill32<<<grid_for(n / 4 + 1, 256), 256, 0, s>>>(p, n, v);
     PCR_HIP(hipGetLastError());
     return PCR_OK;
 }
