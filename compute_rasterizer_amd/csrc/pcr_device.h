@@ -149,18 +149,21 @@ __device__ __forceinline__ void decode(const Decode& d, uint32_t lo, uint32_t me
 // code below is the same operations on the same values, with the reciprocal
 // step shared by the two numerators. Every other operand (a zero or tiny
 // numerator, an extreme depth) takes the compiler's division.
+__device__ __forceinline__ void div_fast(float px, float py, float pw, float& nx, float& ny) {
+    const float y0 = __builtin_amdgcn_rcpf(pw);
+    const float y = fma_(fma_(-pw, y0, 1.0f), y0, y0);
+    float q = px * y;
+    q = fma_(fma_(-pw, q, px), y, q);
+    nx = fma_(fma_(-pw, q, px), y, q);
+    q = py * y;
+    q = fma_(fma_(-pw, q, py), y, q);
+    ny = fma_(fma_(-pw, q, py), y, q);
+}
 __device__ __forceinline__ void div_by_w(float px, float py, float pw, float& nx, float& ny) {
     const float lo = fminf(fminf(fabsf(px), fabsf(py)), pw);
     const float hi = fmaxf(fmaxf(fabsf(px), fabsf(py)), pw);
     if (lo >= 0x1p-40f && hi <= 0x1p40f) {
-        const float y0 = __builtin_amdgcn_rcpf(pw);
-        const float y = fma_(fma_(-pw, y0, 1.0f), y0, y0);
-        float q = px * y;
-        q = fma_(fma_(-pw, q, px), y, q);
-        nx = fma_(fma_(-pw, q, px), y, q);
-        q = py * y;
-        q = fma_(fma_(-pw, q, py), y, q);
-        ny = fma_(fma_(-pw, q, py), y, q);
+        div_fast(px, py, pw, nx, ny);
     } else {
         nx = px / pw;
         ny = py / pw;
@@ -229,6 +232,30 @@ __device__ __forceinline__ bool project_xy(const Proj& p, float x, float y, floa
     return true;
 }
 
+// project_xy() for a batch whose screen_rect() reported `interior`, without
+// its tests. There every point's pw lies in [2^-13, 2^40] and |px|, |py| in
+// [0, 2^40], so the division is div_by_w()'s fast sequence: correctly rounded
+// for a numerator of at least 2^-40, and below that |nx| < 2^-26, where
+// fma(nx, 0.5, 0.5) is exactly 0.5 however the quotient rounds. The NDC and
+// image tests fall to the caller's rectangle test: the rectangle keeps a
+// pixel from every image edge, and a quotient below -1 (above 1) gives a
+// pixel coordinate of -1 or 0 (the width or height) after the clamp. Returns
+// the pixel relative to (x0, y0) as unsigned, so both tests are `< size`.
+// Wherever project_xy() accepts a point, the pixel and depth are its own.
+__device__ __forceinline__ void project_interior(const Proj& p, float x, float y, float z, int x0, int y0,
+                                                 uint32_t& ux, uint32_t& uy, uint32_t& depth_bits) {
+    const float px = fma_(p.t8, z, fma_(p.t4, y, fma_(p.t0, x, p.c12)));
+    const float py = fma_(p.t9, z, fma_(p.t5, y, fma_(p.t1, x, p.c13)));
+    const float pw = fma_(p.t11, z, fma_(p.t7, y, fma_(p.t3, x, p.c15)));
+    float nx, ny;
+    div_fast(px, py, pw, nx, ny);
+    const float sx = __builtin_amdgcn_fmed3f(fma_(nx, 0.5f, 0.5f) * p.fw, -1.0f, p.fw);
+    const float sy = __builtin_amdgcn_fmed3f(fma_(ny, 0.5f, 0.5f) * p.fh, -1.0f, p.fh);
+    ux = (uint32_t)((int)sx - x0);
+    uy = (uint32_t)((int)sy - y0);
+    depth_bits = __float_as_uint(pw);
+}
+
 // Screen rectangle (inclusive, clamped to the image, 1-pixel margin) covered
 // by the projection of an AABB; false when a corner is not in front of the
 // eye or the rectangle is empty. It only routes points to a tile: a point
@@ -239,8 +266,17 @@ __device__ __forceinline__ bool project_xy(const Proj& p, float x, float y, floa
 // one corner, and the 8-lane groups reduce with xor shuffles, so a wave
 // spends one corner's arithmetic instead of eight (the rectangle is computed
 // once per batch by every wave of the workgroup).
+//
+// `interior` (optional) reports whether project_interior() may replace
+// project_xy() for the batch's points: the rectangle keeps a pixel from
+// every image edge, each row of the transform has terms of at most 2^38 in
+// sum over the box (ex, ey, ew), and every corner's pw is at least
+// 2^-12 + 2^-16 * ew. A decoded point lies within a few float steps of the
+// box, and its computed pw within about 10 * 2^-24 * ew of the smallest
+// corner's, so it keeps pw >= 2^-13, and |px|, |py|, pw <= 2^40.
 __device__ inline bool screen_rect(const float* __restrict__ t, const float mn[3], const float mx[3], int w, int h,
-                                   int& x0, int& y0, int& x1, int& y1) {
+                                   int& x0, int& y0, int& x1, int& y1, bool* interior = nullptr) {
+    if (interior) *interior = false;
     const int c = (int)(__lane_id() & 7u);
     const float x = (c & 1) ? mx[0] : mn[0], y = (c & 2) ? mx[1] : mn[1], z = (c & 4) ? mx[2] : mn[2];
     const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
@@ -262,6 +298,15 @@ __device__ inline bool screen_rect(const float* __restrict__ t, const float mn[3
     sx1 = fminf(sx1, (float)w + 2.0f); sy1 = fminf(sy1, (float)h + 2.0f);
     x0 = max((int)floorf(sx0) - 1, 0); y0 = max((int)floorf(sy0) - 1, 0);
     x1 = min((int)floorf(sx1) + 1, w - 1); y1 = min((int)floorf(sy1) + 1, h - 1);
+    if (interior) {
+        const float ax = fmaxf(fabsf(mn[0]), fabsf(mx[0])), ay = fmaxf(fabsf(mn[1]), fabsf(mx[1])),
+                    az = fmaxf(fabsf(mn[2]), fabsf(mx[2]));
+        const float ex = fabsf(t[12]) + fabsf(t[0]) * ax + fabsf(t[4]) * ay + fabsf(t[8]) * az;
+        const float ey = fabsf(t[13]) + fabsf(t[1]) * ax + fabsf(t[5]) * ay + fabsf(t[9]) * az;
+        const float ew = fabsf(t[15]) + fabsf(t[3]) * ax + fabsf(t[7]) * ay + fabsf(t[11]) * az;
+        const bool ok = ex <= 0x1p38f && ey <= 0x1p38f && ew <= 0x1p38f && pw >= 0x1p-12f + 0x1p-16f * ew;
+        *interior = __ballot(!ok) == 0ull && x0 >= 1 && y0 >= 1 && x1 <= w - 2 && y1 <= h - 2;
+    }
     return x1 >= x0 && y1 >= y0;
 }
 

@@ -326,7 +326,11 @@ int launch_selftest_divide(hipStream_t s, const float* nx, const float* ny, cons
 // usual compare + global atomicMin. min is associative and commutative, so
 // the framebuffer is bit-identical to the direct path; the per-point global
 // traffic drops to one access per touched pixel per batch.
-template <int LC, bool TILE>
+//
+// INTERIOR (a batch for which screen_rect() reported `interior`): points
+// project without project_xy()'s tests, and one that lands in the tile is
+// done; any other point runs the full test below, so the result is the same.
+template <int LC, bool TILE, bool INTERIOR = false>
 __device__ __forceinline__ void tile_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi, uint32_t local,
                                            uint32_t np, uint32_t first, const Proj& pj,
                                            unsigned long long* __restrict__ fb, unsigned long long* tile, int x0,
@@ -334,6 +338,15 @@ __device__ __forceinline__ void tile_point(const Decode& d, uint32_t lo, uint32_
     if (local >= np) return;
     float p[3];
     decode<LC>(d, lo, med, hi, p);
+    if (INTERIOR) {
+        uint32_t ux, uy, depth;
+        project_interior(pj, p[0], p[1], p[2], x0, y0, ux, uy, depth);
+        if (ux < (uint32_t)tw && uy < (uint32_t)th) {
+            // uy < th and tw are at most the tile size, so 24-bit products are exact
+            lds_min(tile + __umul24(uy, (uint32_t)tw) + ux, ((unsigned long long)depth << 32) | (first + local));
+            return;
+        }
+    }
     int ix, iy;
     uint32_t depth;
     if (!project_xy(pj, p[0], p[1], p[2], ix, iy, depth)) return;
@@ -351,7 +364,7 @@ __device__ __forceinline__ u32x4 ld_plane(const u32x4* p) {
     return *p;
 }
 
-template <int BLOCK, int LC, bool TILE, bool NT>
+template <int BLOCK, int LC, bool TILE, bool NT, bool INTERIOR = false>
 __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __restrict__ lo,
                                           const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                           uint32_t first, uint32_t np, const Proj& pj,
@@ -372,22 +385,28 @@ __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __res
         const u32x4 m = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
         const uint32_t base = q * 4u - lead;
-        tile_point<LC, TILE>(d, a.x, m.x, c.x, base + 0, np, first, pj, fb, tile, x0, y0, tw, th);
-        tile_point<LC, TILE>(d, a.y, m.y, c.y, base + 1, np, first, pj, fb, tile, x0, y0, tw, th);
-        tile_point<LC, TILE>(d, a.z, m.z, c.z, base + 2, np, first, pj, fb, tile, x0, y0, tw, th);
-        tile_point<LC, TILE>(d, a.w, m.w, c.w, base + 3, np, first, pj, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE, INTERIOR>(d, a.x, m.x, c.x, base + 0, np, first, pj, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE, INTERIOR>(d, a.y, m.y, c.y, base + 1, np, first, pj, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE, INTERIOR>(d, a.z, m.z, c.z, base + 2, np, first, pj, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE, INTERIOR>(d, a.w, m.w, c.w, base + 3, np, first, pj, fb, tile, x0, y0, tw, th);
     }
 }
 
+// The interior loop exists for the 10-bit levels only, which draw nearly all
+// points; the 20- and 30-bit levels keep one loop each (code size).
 template <int BLOCK, bool TILE, bool NT>
 __device__ __forceinline__ void tile_dispatch(int level, const Decode& d, const uint32_t* __restrict__ lo,
                                               const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                               const pcr_batch& h, const Proj& pj, unsigned long long* __restrict__ fb,
-                                              unsigned long long* tile, int x0, int y0, int tw, int th) {
+                                              unsigned long long* tile, int x0, int y0, int tw, int th,
+                                              bool interior = false) {
     if (level == 0)
         tile_loop<BLOCK, 0, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th);
     else if (level == 1)
         tile_loop<BLOCK, 1, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th);
+    else if (TILE && interior)
+        tile_loop<BLOCK, 2, true, NT, true>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw,
+                                            th);
     else
         tile_loop<BLOCK, 2, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th);
 }
@@ -402,26 +421,43 @@ __device__ __forceinline__ void tiled_batch(uint32_t bi, int level, const pcr_ba
     const Decode d = make_decode(level, h.min, h.max);
     const Proj pj = make_proj(v.transform, v.width, v.height);
     int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
-    bool use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1);
+    bool interior;
+    bool use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1, &interior);
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
-    use = use && tw <= TILE_PIX && th <= TILE_PIX && tw * th <= TILE_PIX;
+    // A rectangle one column wide (clamped at the image's left or right edge,
+    // where only rounding can put a point inside) draws directly: the tile's
+    // row split needs tw >= 2 (ceil(2^32 / tw) must fit 32 bits).
+    use = use && tw >= 2 && tw <= TILE_PIX && th <= TILE_PIX && tw * th <= TILE_PIX;
     // every thread evaluates `use` from the same header and view, so the
     // branch (and the barriers inside it) is uniform across the workgroup
     if (use) {
         const int n = tw * th;
         for (int i = threadIdx.x; i < n; i += BLOCK) tile[i] = PCR_FB_CLEAR;
         __syncthreads();
-        tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, pj, fb, tile, x0, y0, tw, th);
+        tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, pj, fb, tile, x0, y0, tw, th, interior);
         __syncthreads();
+        // Four touched pixels per thread at a time: their compare loads are
+        // all issued before the first is needed, so a thread waits once per
+        // four pixels (fb_min's compare and atomic, reordered across pixels).
         // row = i / tw as a multiply-high by ceil(2^32 / tw): exact because
-        // i < 2^13 and tw <= 2^13 keep the error below 1/tw
+        // i < 2^13 and 2 <= tw <= 2^13 keep the error below 1/tw
         const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
-        for (int i = threadIdx.x; i < n; i += BLOCK) {
-            const unsigned long long k = tile[i];
-            if (k != PCR_FB_CLEAR) {
+        constexpr int F = 4;
+        for (int i0 = threadIdx.x; i0 < n; i0 += F * BLOCK) {
+            unsigned long long k[F], old[F];
+            uint32_t pix[F];
+#pragma unroll
+            for (int j = 0; j < F; j++) {
+                const int i = i0 + j * BLOCK;
+                k[j] = i < n ? tile[i] : PCR_FB_CLEAR;
                 const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
-                fb_min(fb, (uint32_t)x0 + ux + ((uint32_t)y0 + uy) * (uint32_t)v.width, k);
+                pix[j] = (uint32_t)x0 + ux + ((uint32_t)y0 + uy) * (uint32_t)v.width;
             }
+#pragma unroll
+            for (int j = 0; j < F; j++) old[j] = k[j] != PCR_FB_CLEAR ? fb[pix[j]] : 0ull;
+#pragma unroll
+            for (int j = 0; j < F; j++)
+                if (k[j] < old[j]) atomicMin(fb + pix[j], k[j]);
         }
     } else {
         tile_dispatch<BLOCK, false, NT>(level, d, lo, med, hi, h, pj, fb, nullptr, 0, 0, 0, 0);
@@ -713,7 +749,9 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
     const bool on_screen = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1);
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
     constexpr int cap = HqsTile<PASS, TILE_PIX>::kCap;
-    const bool fits = on_screen && tw <= cap && h.num_points <= kHqsTileMaxPoints;
+    // one-column rectangles draw directly, as in tiled_batch (tw >= 2 keeps
+    // the row split's ceil(2^32 / tw) within 32 bits)
+    const bool fits = on_screen && tw >= 2 && tw <= cap && h.num_points <= kHqsTileMaxPoints;
     const int bh = fits ? min(th, cap / tw) : 1;    // rows per band
     const int nbands = fits ? (th + bh - 1) / bh : 0;
     // bands pay when the batch is dense (a point or more per rectangle pixel,
@@ -763,16 +801,19 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
 
 // Persistent workgroups over the select stage's list, in the spread order of
 // k_raster_tiled; pass 1 takes items from queue_head, pass 2 (launched after
-// pass 1 has finished) from queue_head2.
+// pass 1 has finished) from queue_head2. Pass 1 is held to 7 waves per SIMD
+// (72 registers), which it reaches without spilling.
 template <int BLOCK, int TILE_PIX, int PASS>
-__global__ __launch_bounds__(BLOCK) void k_hqs_tiled(const pcr_batch* __restrict__ b,
-                                                     const uint8_t* __restrict__ levels,
-                                                     const uint32_t* __restrict__ list, FrameState* __restrict__ st,
-                                                     const uint32_t* __restrict__ lo,
-                                                     const uint32_t* __restrict__ med,
-                                                     const uint32_t* __restrict__ hi,
-                                                     const uint32_t* __restrict__ rgba, pcr_view v,
-                                                     uint32_t* __restrict__ depth, uint32_t* __restrict__ acc) {
+__global__ __launch_bounds__(BLOCK, PASS == 0 ? 7 : 1) void k_hqs_tiled(const pcr_batch* __restrict__ b,
+                                                                        const uint8_t* __restrict__ levels,
+                                                                        const uint32_t* __restrict__ list,
+                                                                        FrameState* __restrict__ st,
+                                                                        const uint32_t* __restrict__ lo,
+                                                                        const uint32_t* __restrict__ med,
+                                                                        const uint32_t* __restrict__ hi,
+                                                                        const uint32_t* __restrict__ rgba, pcr_view v,
+                                                                        uint32_t* __restrict__ depth,
+                                                                        uint32_t* __restrict__ acc) {
     __shared__ HqsTile<PASS, TILE_PIX> tile;
     __shared__ uint32_t s_item;
     uint32_t* head = PASS == 0 ? &st->queue_head : &st->queue_head2;
