@@ -801,19 +801,20 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
 
 // Persistent workgroups over the select stage's list, in the spread order of
 // k_raster_tiled; pass 1 takes items from queue_head, pass 2 (launched after
-// pass 1 has finished) from queue_head2. Pass 1 is held to 7 waves per SIMD
-// (72 registers), which it reaches without spilling.
+// pass 1 has finished) from queue_head2. Pass 1 with tiles up to 2048 px is
+// held to 7 waves per SIMD (72 registers), which it reaches without spilling.
 template <int BLOCK, int TILE_PIX, int PASS>
-__global__ __launch_bounds__(BLOCK, PASS == 0 ? 7 : 1) void k_hqs_tiled(const pcr_batch* __restrict__ b,
-                                                                        const uint8_t* __restrict__ levels,
-                                                                        const uint32_t* __restrict__ list,
-                                                                        FrameState* __restrict__ st,
-                                                                        const uint32_t* __restrict__ lo,
-                                                                        const uint32_t* __restrict__ med,
-                                                                        const uint32_t* __restrict__ hi,
-                                                                        const uint32_t* __restrict__ rgba, pcr_view v,
-                                                                        uint32_t* __restrict__ depth,
-                                                                        uint32_t* __restrict__ acc) {
+__global__ __launch_bounds__(BLOCK, PASS == 0 && TILE_PIX <= 2048 ? 7 : 1)
+void k_hqs_tiled(const pcr_batch* __restrict__ b,
+                 const uint8_t* __restrict__ levels,
+                 const uint32_t* __restrict__ list,
+                 FrameState* __restrict__ st,
+                 const uint32_t* __restrict__ lo,
+                 const uint32_t* __restrict__ med,
+                 const uint32_t* __restrict__ hi,
+                 const uint32_t* __restrict__ rgba, pcr_view v,
+                 uint32_t* __restrict__ depth,
+                 uint32_t* __restrict__ acc) {
     __shared__ HqsTile<PASS, TILE_PIX> tile;
     __shared__ uint32_t s_item;
     uint32_t* head = PASS == 0 ? &st->queue_head : &st->queue_head2;
