@@ -638,7 +638,27 @@ struct TileBand {
     bool first;
 };
 
-template <int PASS, int LC, bool TILE>
+// A point in the band's rows of the rectangle: its tile pixel (pass 1) or,
+// if visible, its sums (pass 2).
+template <int PASS>
+__device__ __forceinline__ void hqs_band_point(uint32_t ux, uint32_t ry, uint32_t dep, uint32_t color,
+                                               uint32_t* tdepth, unsigned long long* tacc, const TileBand& tb) {
+    const uint32_t by = ry - (uint32_t)tb.boff;
+    if (by >= (uint32_t)tb.bh) return;
+    const uint32_t t = by * (uint32_t)tb.tw + ux;
+    if (PASS == 0) {
+        lds_min32(tdepth + t, dep);
+    } else {
+        if (!(__uint_as_float(dep) <= __uint_as_float(tdepth[t]) * 1.01f)) return;
+        lds_add64(tacc + 2 * t, (unsigned long long)(color & 255u) | ((unsigned long long)((color >> 8) & 255u) << 32));
+        lds_add64(tacc + 2 * t + 1, (unsigned long long)((color >> 16) & 255u) | (1ull << 32));
+    }
+}
+
+// INTERIOR: as in tile_point, a batch whose rectangle screen_rect() reported
+// `interior` projects without project_xy()'s tests; a point outside the
+// rectangle runs them below.
+template <int PASS, int LC, bool TILE, bool INTERIOR = false>
 __device__ __forceinline__ void hqs_tile_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi,
                                                uint32_t color, uint32_t local, uint32_t np, const Proj& pj,
                                                uint32_t* __restrict__ depth, uint32_t* __restrict__ acc,
@@ -646,23 +666,21 @@ __device__ __forceinline__ void hqs_tile_point(const Decode& d, uint32_t lo, uin
     if (local >= np) return;
     float p[3];
     decode<LC>(d, lo, med, hi, p);
+    if (INTERIOR) {
+        uint32_t ux, ry, dep;
+        project_interior(pj, p[0], p[1], p[2], tb.x0, tb.y0, ux, ry, dep);
+        if (ux < (uint32_t)tb.tw && ry < (uint32_t)tb.th) {
+            hqs_band_point<PASS>(ux, ry, dep, color, tdepth, tacc, tb);
+            return;
+        }
+    }
     int ix, iy;
     uint32_t dep;
     if (!project_xy(pj, p[0], p[1], p[2], ix, iy, dep)) return;
     if (TILE) {
         const uint32_t ux = (uint32_t)(ix - tb.x0), ry = (uint32_t)(iy - tb.y0);
         if (ux < (uint32_t)tb.tw && ry < (uint32_t)tb.th) {
-            const uint32_t by = ry - (uint32_t)tb.boff;
-            if (by >= (uint32_t)tb.bh) return;
-            const uint32_t t = by * (uint32_t)tb.tw + ux;
-            if (PASS == 0) {
-                lds_min32(tdepth + t, dep);
-            } else {
-                if (!(__uint_as_float(dep) <= __uint_as_float(tdepth[t]) * 1.01f)) return;
-                lds_add64(tacc + 2 * t,
-                          (unsigned long long)(color & 255u) | ((unsigned long long)((color >> 8) & 255u) << 32));
-                lds_add64(tacc + 2 * t + 1, (unsigned long long)((color >> 16) & 255u) | (1ull << 32));
-            }
+            hqs_band_point<PASS>(ux, ry, dep, color, tdepth, tacc, tb);
             return;
         }
         if (!tb.first) return;
@@ -672,7 +690,7 @@ __device__ __forceinline__ void hqs_tile_point(const Decode& d, uint32_t lo, uin
     else hqs_accum(depth, acc, pix, dep, color);
 }
 
-template <int BLOCK, int PASS, int LC, bool TILE, bool NT>
+template <int BLOCK, int PASS, int LC, bool TILE, bool NT, bool INTERIOR = false>
 __device__ __forceinline__ void hqs_tile_loop(const Decode& d, const uint32_t* __restrict__ lo,
                                               const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                               const uint32_t* __restrict__ rgba, uint32_t first, uint32_t np,
@@ -691,10 +709,14 @@ __device__ __forceinline__ void hqs_tile_loop(const Decode& d, const uint32_t* _
         const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 col = PASS == 1 ? ld_plane<NT>(c4 + q) : u32x4{0, 0, 0, 0};
         const uint32_t base = q * 4u - lead;
-        hqs_tile_point<PASS, LC, TILE>(d, a.x, m.x, c.x, col.x, base + 0, np, pj, depth, acc, tdepth, tacc, tb);
-        hqs_tile_point<PASS, LC, TILE>(d, a.y, m.y, c.y, col.y, base + 1, np, pj, depth, acc, tdepth, tacc, tb);
-        hqs_tile_point<PASS, LC, TILE>(d, a.z, m.z, c.z, col.z, base + 2, np, pj, depth, acc, tdepth, tacc, tb);
-        hqs_tile_point<PASS, LC, TILE>(d, a.w, m.w, c.w, col.w, base + 3, np, pj, depth, acc, tdepth, tacc, tb);
+        hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.x, m.x, c.x, col.x, base + 0, np, pj, depth, acc, tdepth,
+                                                 tacc, tb);
+        hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.y, m.y, c.y, col.y, base + 1, np, pj, depth, acc, tdepth,
+                                                 tacc, tb);
+        hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.z, m.z, c.z, col.z, base + 2, np, pj, depth, acc, tdepth,
+                                                 tacc, tb);
+        hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.w, m.w, c.w, col.w, base + 3, np, pj, depth, acc, tdepth,
+                                                 tacc, tb);
     }
 }
 
@@ -704,13 +726,17 @@ __device__ __forceinline__ void hqs_tile_dispatch(int level, const Decode& d, co
                                                   const uint32_t* __restrict__ rgba, const pcr_batch& h,
                                                   const Proj& pj, uint32_t* __restrict__ depth,
                                                   uint32_t* __restrict__ acc, uint32_t* tdepth,
-                                                  unsigned long long* tacc, const TileBand& tb) {
+                                                  unsigned long long* tacc, const TileBand& tb,
+                                                  bool interior = false) {
     if (level == 0)
         hqs_tile_loop<BLOCK, PASS, 0, TILE, NT>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth, acc,
                                                 tdepth, tacc, tb);
     else if (level == 1)
         hqs_tile_loop<BLOCK, PASS, 1, TILE, NT>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth, acc,
                                                 tdepth, tacc, tb);
+    else if (TILE && interior)
+        hqs_tile_loop<BLOCK, PASS, 2, true, NT, true>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth,
+                                                      acc, tdepth, tacc, tb);
     else
         hqs_tile_loop<BLOCK, PASS, 2, TILE, NT>(d, lo, med, hi, rgba, h.first_point, h.num_points, pj, depth, acc,
                                                 tdepth, tacc, tb);
@@ -746,7 +772,8 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
     const Decode d = make_decode(level, h.min, h.max);
     const Proj pj = make_proj(v.transform, v.width, v.height);
     int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
-    const bool on_screen = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1);
+    bool interior;
+    const bool on_screen = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1, &interior);
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
     constexpr int cap = HqsTile<PASS, TILE_PIX>::kCap;
     // one-column rectangles draw directly, as in tiled_batch (tw >= 2 keeps
@@ -782,7 +809,8 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
         }
         __syncthreads();
         const TileBand tb{x0, y0, tw, th, boff, rows, k == 0};
-        hqs_tile_dispatch<BLOCK, PASS, true, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, tdepth, tacc, tb);
+        hqs_tile_dispatch<BLOCK, PASS, true, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, tdepth, tacc, tb,
+                                                 interior);
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += BLOCK) {
             const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
