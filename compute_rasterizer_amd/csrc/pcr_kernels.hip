@@ -781,8 +781,14 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
                                                 const uint32_t* __restrict__ hi, const uint32_t* __restrict__ rgba,
                                                 const pcr_view& v, uint32_t* __restrict__ depth,
                                                 uint32_t* __restrict__ acc, uint32_t* tdepth,
-                                                unsigned long long* tacc) {
+                                                unsigned long long* tacc, uint32_t part = 0, uint32_t nparts = 1) {
     const pcr_batch h = b[bi];
+    pcr_batch hp = h;   // the points this item draws; the tile and band choices follow the whole batch
+    if (nparts > 1) {   // points [part * k, (part + 1) * k), k = ceil(n / nparts)
+        const uint32_t k = (h.num_points + nparts - 1) / nparts, off = min(part * k, h.num_points);
+        hp.first_point += off;
+        hp.num_points = min(k, h.num_points - off);
+    }
     const Decode d = make_decode(level, h.min, h.max);
     const Proj pj = make_proj(v.transform, v.width, v.height);
     int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
@@ -800,7 +806,7 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
     // node, costs more to re-project per band than its few atomics do
     if (!fits || nbands > kHqsMaxBands || (nbands > 1 && (uint64_t)h.num_points < (uint64_t)tw * th)) {
         const TileBand none{0, 0, 0, 0, 0, 0, true};
-        hqs_tile_dispatch<BLOCK, PASS, false, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, nullptr, nullptr,
+        hqs_tile_dispatch<BLOCK, PASS, false, NT>(level, d, lo, med, hi, rgba, hp, pj, depth, acc, nullptr, nullptr,
                                                   none);
         return;
     }
@@ -823,7 +829,7 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
         }
         __syncthreads();
         const TileBand tb{x0, y0, tw, th, boff, rows, k == 0};
-        hqs_tile_dispatch<BLOCK, PASS, true, NT>(level, d, lo, med, hi, rgba, h, pj, depth, acc, tdepth, tacc, tb,
+        hqs_tile_dispatch<BLOCK, PASS, true, NT>(level, d, lo, med, hi, rgba, hp, pj, depth, acc, tdepth, tacc, tb,
                                                  interior);
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += BLOCK) {
@@ -861,16 +867,20 @@ void k_hqs_tiled(const pcr_batch* __restrict__ b,
     __shared__ uint32_t s_item;
     uint32_t* head = PASS == 0 ? &st->queue_head : &st->queue_head2;
     const uint32_t count = st->list_count;
+    const uint32_t nsplit = min(count, gridDim.x), whole = count - nsplit;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
         __syncthreads();
-        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
-        if (item >= count) break;
+        uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= count + nsplit) break;
+        // as in k_raster_tiled, the last item per workgroup is drawn as two halves
+        uint32_t part = 0, nparts = 1;
+        if (item >= whole) { part = (item - whole) & 1u; nparts = 2; item = whole + ((item - whole) >> 1); }
         const uint32_t slot = (uint32_t)(((uint64_t)item * 2654435761ull) % count);
         const uint32_t bi = __builtin_amdgcn_readfirstlane(list[slot]);
         const int level = __builtin_amdgcn_readfirstlane(levels[bi]);
         hqs_tiled_batch<BLOCK, TILE_PIX, PASS, true>(bi, level, b, lo, med, hi, rgba, v, depth, acc, tile.depth,
-                                                      tile.acc);
+                                                      tile.acc, part, nparts);
         __syncthreads();   // s_item and the tile are reused by the next item
     }
 }
@@ -934,11 +944,15 @@ __global__ __launch_bounds__(BLOCK) void k_mv_hqs_tiled(const pcr_batch* __restr
     __shared__ uint8_t s_vid[64], s_lv[64];
     uint32_t* head = PASS == 0 ? &st->queue_head : &st->queue_head2;
     const uint32_t count = st->list_count;
+    const uint32_t nsplit = min(count, gridDim.x), whole = count - nsplit;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
         __syncthreads();
-        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
-        if (item >= count) break;
+        uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= count + nsplit) break;
+        // as in k_hqs_tiled, the last item per workgroup is drawn as two halves
+        uint32_t part = 0, nparts = 1;
+        if (item >= whole) { part = (item - whole) & 1u; nparts = 2; item = whole + ((item - whole) >> 1); }
         const uint32_t bi = __builtin_amdgcn_readfirstlane(list[item]);
         if (threadIdx.x < 64) {   // compact the views drawing this batch (wave 0)
             const int v = (int)threadIdx.x;
@@ -957,7 +971,7 @@ __global__ __launch_bounds__(BLOCK) void k_mv_hqs_tiled(const pcr_batch* __restr
             const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
             const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
             hqs_tiled_batch<BLOCK, TILE_PIX, PASS, false>(bi, lv, b, lo, med, hi, rgba, views[vi], depths[vi],
-                                                           accs[vi], tile.depth, tile.acc);
+                                                           accs[vi], tile.depth, tile.acc, part, nparts);
             __syncthreads();   // the tile is reused by the next view
         }
     }
