@@ -561,11 +561,15 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster_tiled(const pcr_batch* __re
     __shared__ uint32_t s_item, s_nv;
     __shared__ uint8_t s_vid[64], s_lv[64];
     const uint32_t count = st->list_count;
+    const uint32_t nsplit = min(count, gridDim.x), whole = count - nsplit;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(&st->queue_head, 1u);
         __syncthreads();
-        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
-        if (item >= count) break;
+        uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= count + nsplit) break;
+        // as in k_raster_tiled, the last item per workgroup is drawn as two halves
+        uint32_t part = 0, nparts = 1;
+        if (item >= whole) { part = (item - whole) & 1u; nparts = 2; item = whole + ((item - whole) >> 1); }
         const uint32_t bi = __builtin_amdgcn_readfirstlane(list[item]);
         if (threadIdx.x < 64) {   // compact the views drawing this batch (wave 0)
             const int v = (int)threadIdx.x;
@@ -583,7 +587,7 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster_tiled(const pcr_batch* __re
         for (uint32_t k = 0; k < nv; k++) {
             const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
             const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
-            tiled_batch<BLOCK, TILE_PIX, false>(bi, lv, b, lo, med, hi, views[vi], fbs[vi], tile);
+            tiled_batch<BLOCK, TILE_PIX, false>(bi, lv, b, lo, med, hi, views[vi], fbs[vi], tile, part, nparts);
             __syncthreads();   // the tile is reused by the next view
         }
     }
