@@ -352,9 +352,10 @@ __device__ __forceinline__ void hqs_accum(const uint32_t* __restrict__ depth, ui
     if (!(__uint_as_float(d) <= bd * 1.01f)) return;
     add_pairs(acc + 4ull * pix, color & 255u, (color >> 8) & 255u, (color >> 16) & 255u, 1u);
 }
-// hqs/resolve.cs:74-129: integer averages, background where count == 0.
+// hqs/resolve.cs:74-129: integer averages r | g << 8 | b << 16 with alpha 0
+// (resolve.cs:119), the background where count == 0.
 __device__ __forceinline__ uint32_t hqs_average(const u32x4 a, uint32_t bg) {
-    return a.w == 0 ? bg : ((a.x / a.w) | ((a.y / a.w) << 8) | ((a.z / a.w) << 16) | 0xFF000000u);
+    return a.w == 0 ? bg : ((a.x / a.w) | ((a.y / a.w) << 8) | ((a.z / a.w) << 16));
 }
 // compute_loop_nodes_hqs/resolve.cs:92-158: a float average over the
 // (2*window+1)^2 neighbourhood of pixel (x, y), weights 100 (centre),

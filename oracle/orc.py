@@ -48,12 +48,13 @@ def load():
         f.orc_precision_level.restype = C.c_int
         f.orc_precision_level.argtypes = [P, P, C.POINTER(View), C.c_float]
         f.orc_decode.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P]
+        f.orc_decode_n.argtypes = [C.c_int, C.c_uint64, P, P, P, P, P, P]
         f.orc_project.restype = C.c_int
         f.orc_project.argtypes = [P, P, C.c_int, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         f.orc_clear64.argtypes = [P, C.c_uint64]
         f.orc_render_points16.argtypes = [P, P, C.c_uint64, C.POINTER(View), P]
         f.orc_render_batched.argtypes = [P, C.c_uint32, P, P, P, C.POINTER(View), C.c_int, P, P]
-        f.orc_resolve_index.argtypes = [P, C.c_uint64, P, C.c_uint32, P]
+        f.orc_resolve_index.argtypes = [P, C.c_uint64, P, C.c_uint64, C.c_uint32, P]
         f.orc_resolve_color.argtypes = [P, C.c_uint64, C.c_uint32, P]
         f.orc_edl_exp.argtypes = [C.c_float]
         f.orc_edl_exp.restype = C.c_float
@@ -141,6 +142,16 @@ def decode(level, lo, med, hi, mn, mx):
     return out
 
 
+def decode_n(level, lo, med, hi, mn, mx):
+    """orc_decode over arrays of plane words sharing one box: (n, 3) float32."""
+    lo = np.ascontiguousarray(lo, np.uint32)
+    med = None if med is None else np.ascontiguousarray(med, np.uint32)
+    hi = None if hi is None else np.ascontiguousarray(hi, np.uint32)
+    out = np.zeros((lo.size, 3), np.float32)
+    load().orc_decode_n(level, lo.size, _p(lo), _p(med), _p(hi), _p(f32(mn)), _p(f32(mx)), _p(out))
+    return out
+
+
 def project(t, p, w, h):
     pix, d = C.c_uint32(), C.c_uint32()
     ok = load().orc_project(_p(f32(t)), _p(f32(p)), w, h, C.byref(pix), C.byref(d))
@@ -173,8 +184,10 @@ def render_batched_mt(b, lo, med, hi, view: View, culling=1, nthreads=8, fb=None
 
 
 def resolve_index(fb, rgba, bg):
+    rgba = np.ascontiguousarray(rgba, np.uint32)
+    fb = np.ascontiguousarray(fb, np.uint64)
     out = np.zeros(fb.shape[0], np.uint32)
-    load().orc_resolve_index(_p(fb), fb.shape[0], _p(np.ascontiguousarray(rgba, np.uint32)), bg, _p(out))
+    load().orc_resolve_index(_p(fb), fb.shape[0], _p(rgba), rgba.shape[0], bg, _p(out))
     return out
 
 

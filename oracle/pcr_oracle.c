@@ -581,6 +581,12 @@ void orc_decode(int level, uint32_t lo, uint32_t med, uint32_t hi,
     }
 }
 
+/* orc_decode over n points sharing one box (out: 3 floats per point) */
+void orc_decode_n(int level, uint64_t n, const uint32_t* lo, const uint32_t* med, const uint32_t* hi,
+                  const float mn[3], const float mx[3], float* out) {
+    for (uint64_t i = 0; i < n; i++) orc_decode(level, lo[i], med ? med[i] : 0u, hi ? hi[i] : 0u, mn, mx, out + 3 * i);
+}
+
 /* render.cs:400-409 with the edge decision (pixel == width/height rejected) */
 int orc_project(const float t[16], const float p[3], int w, int h, uint32_t* pixel, uint32_t* depth_bits) {
     float px = fmaf(t[8], p[2], fmaf(t[4], p[1], fmaf(t[0], p[0], t[12])));
@@ -643,9 +649,16 @@ void orc_render_batched(const pcr_batch* b, uint32_t nb, const uint32_t* lo, con
     }
 }
 
-/* compute_loop_las/resolve.cs:100-146: colour gathered by point index */
-void orc_resolve_index(const uint64_t* fb, uint64_t npix, const uint32_t* rgba, uint32_t bg, uint32_t* out) {
-    for (uint64_t i = 0; i < npix; i++) out[i] = fb[i] == PCR_FB_CLEAR ? bg : rgba[(uint32_t)fb[i]];
+/* compute_loop_las/resolve.cs:100-146: the colour of the pixel's point,
+ * gathered by the index in the key's low 32 bits (:137), all 32 bits kept.
+ * Background iff the key is the clear value (point index 0 is a point and no
+ * depth is capped), or the index is outside the cloud. */
+void orc_resolve_index(const uint64_t* fb, uint64_t npix, const uint32_t* rgba, uint64_t npoints, uint32_t bg,
+                       uint32_t* out) {
+    for (uint64_t i = 0; i < npix; i++) {
+        const uint32_t id = (uint32_t)fb[i];
+        out[i] = (fb[i] == PCR_FB_CLEAR || id >= npoints) ? bg : rgba[id];
+    }
 }
 /* e^x for x <= 0, for the eye-dome lighting shade. It is not libm's expf:
  * the device must produce the same bits, so both sides run this sequence.
@@ -754,13 +767,14 @@ void orc_hqs_points16(const float* xyz, const uint32_t* rgba, uint64_t n, const 
     for (uint64_t i = 0; i < n; i++) hqs_depth_point(v, xyz + 3 * i, depth);
     for (uint64_t i = 0; i < n; i++) hqs_color_point(v, xyz + 3 * i, rgba[i], depth, acc);
 }
-/* hqs/resolve.cs:74-129 (covered pixels get alpha 255) */
+/* hqs/resolve.cs:74-129: truncating integer averages, r | g << 8 | b << 16
+ * with alpha 0 (resolve.cs:119); bg where count == 0 (:121-122) */
 void orc_hqs_resolve(const uint32_t* acc, uint64_t npix, uint32_t bg, uint32_t* out) {
     for (uint64_t i = 0; i < npix; i++) {
         uint32_t c = acc[4 * i + 3];
         if (c == 0) { out[i] = bg; continue; }
         uint32_t r = acc[4 * i] / c, g = acc[4 * i + 1] / c, bb = acc[4 * i + 2] / c;
-        out[i] = r | (g << 8) | (bb << 16) | 0xFF000000u;
+        out[i] = r | (g << 8) | (bb << 16);
     }
 }
 

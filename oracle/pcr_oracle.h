@@ -73,6 +73,8 @@ float orc_projected_px(const float mn[3], const float mx[3], float len,
 int   orc_precision_level(const float mn[3], const float mx[3], const pcr_view* v, float first_threshold);
 void  orc_decode(int level, uint32_t lo, uint32_t med, uint32_t hi,
                  const float mn[3], const float mx[3], float out[3]);
+void  orc_decode_n(int level, uint64_t n, const uint32_t* lo, const uint32_t* med, const uint32_t* hi,
+                   const float mn[3], const float mx[3], float* out);
 /* project a point: returns 1 and pixel index/depth bits if it lands inside */
 int   orc_project(const float t[16], const float p[3], int w, int h, uint32_t* pixel, uint32_t* depth_bits);
 
@@ -84,7 +86,8 @@ void orc_render_points16(const float* xyz, const uint32_t* rgba, uint64_t n,
 void orc_render_batched(const pcr_batch* b, uint32_t nb, const uint32_t* lo, const uint32_t* med,
                         const uint32_t* hi, const pcr_view* v, int frustum_culling,
                         uint64_t* fb, uint8_t* levels_out);
-void orc_resolve_index(const uint64_t* fb, uint64_t npix, const uint32_t* rgba, uint32_t bg, uint32_t* out);
+void orc_resolve_index(const uint64_t* fb, uint64_t npix, const uint32_t* rgba, uint64_t npoints, uint32_t bg,
+                       uint32_t* out);
 void orc_resolve_color(const uint64_t* fb, uint64_t npix, uint32_t bg, uint32_t* out);
 /* eye-dome lighting resolve (compute_loop_las/resolve.cs:148-191, switched on) */
 float orc_edl_exp(float x);

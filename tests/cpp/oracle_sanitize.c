@@ -51,7 +51,7 @@ int main(void) {
     orc_render_batched_mt(b, nb, lo, med, hi, &v, 1, fb2, 4);
     CHECK(memcmp(fb, fb2, 8 * npix) == 0);
     uint32_t* img = malloc(4 * npix);
-    orc_resolve_index(fb, npix, rgba, 0x00443322u, img);
+    orc_resolve_index(fb, npix, rgba, n, 0x00443322u, img);
 
     orc_clear64(fb, npix);
     orc_render_points16(xyz, rgba, n, &v, fb);

@@ -17,13 +17,18 @@ def test_window0_equals_integer_division():
     w, h = 64, 48
     cnt = rng.integers(0, 1000, w * h).astype(np.uint32)
     acc = np.zeros(4 * w * h, np.uint32)
+    sums = []
     for k in range(3):
-        acc[k::4] = (rng.integers(0, 256, w * h) * cnt).astype(np.uint32) + rng.integers(0, 1, w * h).astype(np.uint32)
-        acc[k::4] = np.minimum(acc[k::4], cnt * 255)
+        # a multiple of the count plus a remainder below it: averages that are not integers
+        s = rng.integers(0, 255, w * h) * cnt + rng.integers(0, np.maximum(cnt, 1))
+        acc[k::4] = s.astype(np.uint32)
+        sums.append(s)
     acc[3::4] = cnt
     depth = rng.integers(0x3F800000, 0x44000000, w * h).astype(np.uint32)
     out = orc.hqs_resolve_window(depth, acc, w, h, 0)
-    ref = np.where(cnt > 0, orc.hqs_resolve(acc, 0) & 0x00FFFFFF, 0)
+    c = np.maximum(cnt, 1).astype(np.int64)
+    ref = np.where(cnt > 0, sums[0] // c | (sums[1] // c) << 8 | (sums[2] // c) << 16, 0)
+    assert ((sums[0] % c) != 0).mean() > 0.9
     assert np.array_equal(out, ref)
 
 

@@ -330,7 +330,8 @@ def test_hqs_single_point_pixels_equal_plain_resolve():
     plain = orc.resolve_color(orc.render_points16(xyz, rgba, v), 0x00443322)
     single = acc[3::4] == 1
     assert single.sum() >= 45
-    assert np.array_equal(orc.hqs_resolve(acc, 0x00443322)[single], plain[single])
+    # the hqs image carries r | g << 8 | b << 16 with alpha 0 (hqs/resolve.cs:119)
+    assert np.array_equal(orc.hqs_resolve(acc, 0x00443322)[single], plain[single] & 0x00FFFFFF)
 
 
 # ----------------------------------------------------------------- LOD
