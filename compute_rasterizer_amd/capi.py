@@ -24,6 +24,7 @@ PCR_E_NOT_READY = -6
 PCR_E_CANCELLED = -7
 
 POINTS_PER_BATCH = 10240
+STATIC_MAX_BATCHES = 262144
 FB_CLEAR = 0xFFFFFFFFFFFFFFFF
 DEPTH_CLEAR = 0xFFFFFFFF
 BG_LAS = 0x00443322
@@ -165,6 +166,7 @@ SIGNATURES = {
     "pcr_fb_device_rgba": (P, [P]),
     "pcr_render": (C.c_int, [P, P, C.POINTER(View), C.POINTER(RenderParams), P]),
     "pcr_resolve": (C.c_int, [P, P, C.POINTER(RenderParams), P]),
+    "pcr_render_static": (C.c_int, [P, u32, P, P, P, u64, C.POINTER(View), C.POINTER(RenderParams), P, P]),
     "pcr_stats_get": (C.c_int, [P, C.POINTER(Stats)]),
     "pcr_las_parse_header": (C.c_int, [P, C.c_uint64, P]),
     "pcr_cloud_from_las": (C.c_int, [P, P, C.c_int, P, C.POINTER(C.c_void_p)]),

@@ -391,6 +391,31 @@ int pcr_frame_times(pcr_ctx* ctx, int n, float* out5, int* got);
 int pcr_selftest_divide(pcr_ctx* ctx, const float* num_x, const float* num_y, const float* den, uint64_t n,
                         float* out2);
 
+/* ---- allocation-free render ---------------------------------------------- */
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. Renders one frame exactly as pcr_render (and, with
+ * clear_before, the clear) does for modes LAS, LAS2 and NODES, over a batch
+ * layout and a key framebuffer the caller already holds in device memory, on
+ * the caller's stream of the current HIP device. Allocates nothing: the
+ * frame's scratch (counters, the drawn-batch list and per-batch levels) is a
+ * fixed device array of the library, one per GPU (about 1.3 MB, held until the
+ * process ends), sized for PCR_STATIC_MAX_BATCHES batches; larger layouts are
+ * refused with PCR_E_INVALID. Every call on a GPU shares that GPU's array, so
+ * calls that overlap on one GPU race: serialise them (one stream, or events).
+ *   batches_dev     nbatches headers (as pcr_cloud_upload_batched takes them)
+ *   lo/med/hi_dev   position planes of plane_words words each: 16-byte
+ *                   aligned, plane_words a multiple of 4 and at least every
+ *                   batch's last point + 1 rounded up to 4
+ *   keys_dev        view->width * view->height uint64 keys, 16-byte aligned
+ *   params          time_stages must be 0; mode LAS, LAS2 or NODES
+ *   stream          a hipStream_t of the current device (NULL = null stream) */
+#define PCR_STATIC_MAX_BATCHES 262144u
+int pcr_render_static(const pcr_batch* batches_dev, uint32_t nbatches, const uint32_t* lo_dev,
+                      const uint32_t* med_dev, const uint32_t* hi_dev, uint64_t plane_words,
+                      const pcr_view* view, const pcr_render_params* params, uint64_t* keys_dev,
+                      void* stream);
+
 /* ---- batched multi-view (persistent launch, hipGraph) ------------------- */
 typedef struct pcr_tune {
     int32_t block_size;      /* threads per workgroup: 64,128,256,512            */
