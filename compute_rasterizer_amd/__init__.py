@@ -34,7 +34,7 @@ def look_at(eye, target, up=(0.0, 1.0, 0.0), fovy=60.0, near=0.1, far=1000.0, or
 
 def params(mode=MODE_LAS2, culling=1, background=None, lod_rule=LOD_SPACING, lod_px=1.0, block_size=0,
            clear_before=1, time_stages=0, tile_px=None, batch_begin=0, batch_end=0, resolve_window=0,
-           edl_strength=0.0) -> RenderParams:
+           edl_strength=0.0, occlusion_skip=True, debug_counters=False) -> RenderParams:
     """Render parameters; LAS2, NODES, HQS and NODES_HQS default to the LDS-tiled
     kernels (2048-px tile)."""
     if background is None:
@@ -42,7 +42,8 @@ def params(mode=MODE_LAS2, culling=1, background=None, lod_rule=LOD_SPACING, lod
     if tile_px is None:
         tile_px = 2048 if mode in (MODE_LAS2, MODE_NODES, MODE_HQS, MODE_NODES_HQS) else 0
     return RenderParams(mode, culling, lod_rule, lod_px, background, block_size, clear_before, time_stages,
-                        tile_px, batch_begin, batch_end, resolve_window, edl_strength)
+                        tile_px, batch_begin, batch_end, resolve_window, edl_strength,
+                        0 if occlusion_skip else -1, int(bool(debug_counters)))
 
 
 def potree_info(metadata_json: str) -> capi.PotreeInfo:
@@ -221,6 +222,17 @@ class Server:
         h = C.c_void_p()
         check(capi.load().pcr_server_create(ctx.h, budget_bytes, max_batch, num_streams, C.byref(h)))
         self.ctx, self.h, self.clouds = ctx, h, []
+        self._npix = {}   # ticket -> pixels of its image, recorded at submit
+
+    def _pixels(self, ticket: int, npix):
+        """The ticket's image size; a caller's npix must agree with it (the
+        library copies exactly width * height pixels)."""
+        n = self._npix.get(ticket)
+        if n is None:
+            raise PcrError(capi.PCR_E_INVALID, f"unknown ticket {ticket}")
+        if npix is not None and npix != n:
+            raise ValueError(f"ticket {ticket} has {n} pixels, not {npix}")
+        return n
 
     def add_cloud(self, cloud: Cloud) -> int:
         cid = C.c_int()
@@ -232,38 +244,45 @@ class Server:
         req = capi.Request(cloud_id, mode, lod_px, 0, view)
         t = C.c_uint64()
         check(capi.load().pcr_server_submit(self.h, C.byref(req), C.byref(t)))
+        self._npix[t.value] = view.width * view.height
         return t.value
 
-    def poll(self, ticket: int, npix: int):
+    def poll(self, ticket: int, npix: int | None = None):
         """(image, latency_ms) or None while the request is in flight."""
-        out = np.empty(npix, np.uint32)
+        out = np.empty(self._pixels(ticket, npix), np.uint32)
         lat = C.c_double()
         rc = capi.load().pcr_server_poll(self.h, ticket, ptr(out), C.byref(lat))
         if rc == capi.PCR_E_NOT_READY:
             return None
+        self._npix.pop(ticket, None)
         check(rc)
         return out, lat.value
 
-    def wait(self, ticket: int, npix: int):
-        out = np.empty(npix, np.uint32)
+    def wait(self, ticket: int, npix: int | None = None):
+        out = np.empty(self._pixels(ticket, npix), np.uint32)
         lat = C.c_double()
-        check(capi.load().pcr_server_wait(self.h, ticket, ptr(out), C.byref(lat)))
+        rc = capi.load().pcr_server_wait(self.h, ticket, ptr(out), C.byref(lat))
+        self._npix.pop(ticket, None)
+        check(rc)
         return out, lat.value
 
-    def wait_pinned(self, ticket: int, npix: int):
+    def wait_pinned(self, ticket: int, npix: int | None = None):
         """(image, latency_ms) without a copy: the image is a view of the
         server's pinned result memory, valid until release(ticket)."""
+        n = self._pixels(ticket, npix)
         p = C.c_void_p()
         lat = C.c_double()
         check(capi.load().pcr_server_wait_pinned(self.h, ticket, C.byref(p), C.byref(lat)))
-        img = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), shape=(npix,))
+        img = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), shape=(n,))
         return img, lat.value
 
     def release(self, ticket: int):
         check(capi.load().pcr_server_release(self.h, ticket))
+        self._npix.pop(ticket, None)
 
     def cancel(self, ticket: int):
         check(capi.load().pcr_server_cancel(self.h, ticket))
+        self._npix.pop(ticket, None)
 
     def stats(self) -> capi.ServerStats:
         s = capi.ServerStats()

@@ -62,7 +62,7 @@ class RenderParams(C.Structure):
                 ("lod_px", C.c_float), ("background", C.c_uint32), ("block_size", C.c_int32),
                 ("clear_before", C.c_int32), ("time_stages", C.c_int32), ("tile_px", C.c_int32),
                 ("batch_begin", C.c_uint32), ("batch_end", C.c_uint32), ("resolve_window", C.c_int32),
-                ("edl_strength", C.c_float)]
+                ("edl_strength", C.c_float), ("occlusion_skip", C.c_int32), ("debug_counters", C.c_int32)]
 
 
 class LasInfo(C.Structure):
@@ -86,7 +86,8 @@ class Stats(C.Structure):
     _fields_ = [("batches_total", C.c_uint64), ("batches_culled", C.c_uint64),
                 ("batches_per_level", C.c_uint64 * 5), ("points_drawn", C.c_uint64),
                 ("bytes_algorithmic", C.c_uint64), ("ms_cull", C.c_float), ("ms_raster", C.c_float),
-                ("ms_resolve", C.c_float), ("ms_clear", C.c_float)]
+                ("ms_resolve", C.c_float), ("ms_clear", C.c_float), ("batches_occluded", C.c_uint64),
+                ("points_occluded", C.c_uint64), ("atomics_issued", C.c_uint64), ("pixels_visible", C.c_uint64)]
 
 
 class GenCfg(C.Structure):

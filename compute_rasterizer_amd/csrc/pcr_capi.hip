@@ -228,7 +228,7 @@ int pcr_cloud_upload_points16(pcr_ctx* ctx, const float* xyz, const uint32_t* rg
     }
     c->device_bytes = n * 16;
     for (int a = 0; a < 3; a++) { c->aabb_min[a] = mn[a]; c->aabb_max[a] = mx[a]; }
-    PCR_HIP(hipMemcpy(c->d_points16, packed.data(), n * 16, hipMemcpyHostToDevice));
+    PCR_HIP_OR(hipMemcpy(c->d_points16, packed.data(), n * 16, hipMemcpyHostToDevice), (cloud_free(c), delete c));
     *out = c;
     return PCR_OK;
 }
@@ -259,12 +259,13 @@ int pcr_cloud_upload_batched(pcr_ctx* ctx, const pcr_batch* batches, uint32_t nb
     c->batch_size = batches[0].num_points;
     c->num_levels = max_level + 1;
     for (int a = 0; a < 3; a++) { c->aabb_min[a] = mn[a]; c->aabb_max[a] = mx[a]; }
-    PCR_HIP(hipMemcpy(c->d_batches, batches, sizeof(pcr_batch) * nb, hipMemcpyHostToDevice));
-    PCR_HIP(hipMemcpy(c->d_lo, lo, npoints * 4, hipMemcpyHostToDevice));
-    PCR_HIP(hipMemcpy(c->d_med, med, npoints * 4, hipMemcpyHostToDevice));
-    PCR_HIP(hipMemcpy(c->d_hi, hi, npoints * 4, hipMemcpyHostToDevice));
-    PCR_HIP(hipMemcpy(c->d_rgba, rgba, npoints * 4, hipMemcpyHostToDevice));
-    PCR_HIP(hipStreamSynchronize(ctx->stream));
+    const auto drop = [c] { cloud_free(c); delete c; };
+    PCR_HIP_OR(hipMemcpy(c->d_batches, batches, sizeof(pcr_batch) * nb, hipMemcpyHostToDevice), drop());
+    PCR_HIP_OR(hipMemcpy(c->d_lo, lo, npoints * 4, hipMemcpyHostToDevice), drop());
+    PCR_HIP_OR(hipMemcpy(c->d_med, med, npoints * 4, hipMemcpyHostToDevice), drop());
+    PCR_HIP_OR(hipMemcpy(c->d_hi, hi, npoints * 4, hipMemcpyHostToDevice), drop());
+    PCR_HIP_OR(hipMemcpy(c->d_rgba, rgba, npoints * 4, hipMemcpyHostToDevice), drop());
+    PCR_HIP_OR(hipStreamSynchronize(ctx->stream), drop());
     c->h_batches.assign(batches, batches + nb);
     *out = c;
     return PCR_OK;
@@ -436,6 +437,9 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
               "render: bad lod_rule %d", params->lod_rule);
     PCR_CHECK(params->tile_px == 0 || params->tile_px == 1024 || params->tile_px == 2048 || params->tile_px == 4096,
               "render: tile_px %d not in {0,1024,2048,4096}", params->tile_px);
+    PCR_CHECK(params->occlusion_skip == 0 || params->occlusion_skip == -1, "render: occlusion_skip %d not 0 or -1",
+              params->occlusion_skip);
+    const bool occl = params->occlusion_skip == 0;
     PCR_CHECK(params->batch_end == 0 ? params->batch_begin == 0
                                       : params->batch_begin <= params->batch_end &&
                                             params->batch_end <= (pts16 ? 0u : cloud->num_batches),
@@ -490,12 +494,12 @@ int pcr_render(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* view, const pcr_r
         case PCR_MODE_LAS: rc = launch_raster_batched(s, cloud, *view, 1, block ? block : 128, st, fb->d_keys); break;
         case PCR_MODE_LAS2:
         case PCR_MODE_NODES:
-            rc = params->tile_px ? launch_raster_tiled(s, cloud, *view, params->tile_px, block, st, fb->d_keys)
+            rc = params->tile_px ? launch_raster_tiled(s, cloud, *view, params->tile_px, block, st, fb->d_keys, occl)
                                  : launch_raster_batched(s, cloud, *view, 4, block ? block : 256, st, fb->d_keys);
             break;
         case PCR_MODE_HQS:
         case PCR_MODE_NODES_HQS:
-            rc = params->tile_px ? launch_hqs_tiled(s, cloud, *view, params->tile_px, block, st, fb)
+            rc = params->tile_px ? launch_hqs_tiled(s, cloud, *view, params->tile_px, block, st, fb, occl)
                                  : launch_hqs_batched(s, cloud, *view, st, fb);
             break;
         case PCR_MODE_HQS16: rc = launch_hqs_points16(s, cloud, *view, fb); break;
@@ -521,7 +525,8 @@ int pcr_resolve(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_render_params* params,
     hipEvent_t* ev = ctx->ring[(ctx->frames_timed + PCR_TIMING_RING - 1) % PCR_TIMING_RING];
     if (ctx->timing == 1) PCR_HIP(hipEventRecord(ev[4], ctx->stream));
     int rc = launch_resolve(ctx->stream, cloud, mode, params->background, params->resolve_window,
-                            params->edl_strength, fb);
+                            params->edl_strength, fb,
+                            params->debug_counters == 1 ? ctx->d_state + ctx->last_state : nullptr);
     if (rc) return rc;
     if (ctx->timing == 1) PCR_HIP(hipEventRecord(ev[5], ctx->stream));
     return PCR_OK;
@@ -538,6 +543,10 @@ int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out) {
     for (int i = 0; i < 5; i++) out->batches_per_level[i] = st.level_count[i];
     out->points_drawn = st.points_drawn;
     out->bytes_algorithmic = st.plane_bytes + 64ull * ctx->last_nb;
+    out->batches_occluded = st.occluded_batches;
+    out->points_occluded = st.occluded_points;
+    out->atomics_issued = st.atomics;
+    out->pixels_visible = st.pixels_visible;
     if (ctx->timing && ctx->frames_timed) {
         hipEvent_t* ev = ctx->ring[(ctx->frames_timed - 1) % PCR_TIMING_RING];
         float ms = 0;

@@ -274,9 +274,34 @@ __device__ __forceinline__ void project_interior(const Proj& p, float x, float y
 // 2^-12 + 2^-16 * ew. A decoded point lies within a few float steps of the
 // box, and its computed pw within about 10 * 2^-24 * ew of the smallest
 // corner's, so it keeps pw >= 2^-13, and |px|, |py|, pw <= 2^40.
+//
+// `wlo` (optional, with `interior`) is a depth no point of the batch can
+// undercut, for the occlusion test of rect_open(), or 0 when the test does
+// not apply. It applies to an interior batch whose smallest corner pw is at
+// least max(W, H) * 2^-21 * (max(ex, ey) + ew). Then:
+//   - a decoded point lies in the box (x = fma(X, scale, mn) <= mx, since
+//     X * scale < mx - mn) or, for the 20- and 30-bit levels whose float(X)
+//     rounds up, within an ulp of it;
+//   - its computed pw differs from the affine w at that point, and each
+//     corner's computed pw from w at the corner, by at most 3 roundings of
+//     terms summing to ew, and w over the box is smallest at a corner, so the
+//     point's pw >= min corner pw - 6 * 2^-24 * ew, and wlo = min corner pw -
+//     2^-18 * ew lies below that with a margin of 58 * 2^-24 * ew for the
+//     rounding of the subtraction itself;
+//   - its screen coordinate differs from the exact projection of the point by
+//     at most (W / 2) * (3 * 2^-24 * (ex + ew) / pw + 2^-24) plus the
+//     roundings of fma(nx, 0.5, 0.5) * W (3 * 2^-25 * W), and so do the
+//     corners'; with pw >= W * 2^-21 * (ex + ew) the first term is at most
+//     0.1875 px and all of it below 0.2 px for W <= 32768. The exact
+//     projection of a point of the box lies in the hull of the corners' (all
+//     in front of the eye), so a point's computed coordinate lies within
+//     0.4 px of the corners' computed range, and its pixel inside the
+//     rectangle with its 1-pixel margin (the same holds for y with H).
 __device__ inline bool screen_rect(const float* __restrict__ t, const float mn[3], const float mx[3], int w, int h,
-                                   int& x0, int& y0, int& x1, int& y1, bool* interior = nullptr) {
+                                   int& x0, int& y0, int& x1, int& y1, bool* interior = nullptr,
+                                   float* wlo = nullptr) {
     if (interior) *interior = false;
+    if (wlo) *wlo = 0.0f;
     const int c = (int)(__lane_id() & 7u);
     const float x = (c & 1) ? mx[0] : mn[0], y = (c & 2) ? mx[1] : mn[1], z = (c & 4) ? mx[2] : mn[2];
     const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
@@ -296,8 +321,12 @@ __device__ inline bool screen_rect(const float* __restrict__ t, const float mn[3
     }
     sx0 = fmaxf(sx0, -2.0f); sy0 = fmaxf(sy0, -2.0f);
     sx1 = fminf(sx1, (float)w + 2.0f); sy1 = fminf(sy1, (float)h + 2.0f);
-    x0 = max((int)floorf(sx0) - 1, 0); y0 = max((int)floorf(sy0) - 1, 0);
-    x1 = min((int)floorf(sx1) + 1, w - 1); y1 = min((int)floorf(sy1) + 1, h - 1);
+    // every lane holds the same reduced values: make them wave-uniform (scalar
+    // registers) for the callers' per-batch state
+    x0 = __builtin_amdgcn_readfirstlane(max((int)floorf(sx0) - 1, 0));
+    y0 = __builtin_amdgcn_readfirstlane(max((int)floorf(sy0) - 1, 0));
+    x1 = __builtin_amdgcn_readfirstlane(min((int)floorf(sx1) + 1, w - 1));
+    y1 = __builtin_amdgcn_readfirstlane(min((int)floorf(sy1) + 1, h - 1));
     if (interior) {
         const float ax = fmaxf(fabsf(mn[0]), fabsf(mx[0])), ay = fmaxf(fabsf(mn[1]), fabsf(mx[1])),
                     az = fmaxf(fabsf(mn[2]), fabsf(mx[2]));
@@ -306,16 +335,29 @@ __device__ inline bool screen_rect(const float* __restrict__ t, const float mn[3
         const float ew = fabsf(t[15]) + fabsf(t[3]) * ax + fabsf(t[7]) * ay + fabsf(t[11]) * az;
         const bool ok = ex <= 0x1p38f && ey <= 0x1p38f && ew <= 0x1p38f && pw >= 0x1p-12f + 0x1p-16f * ew;
         *interior = __ballot(!ok) == 0ull && x0 >= 1 && y0 >= 1 && x1 <= w - 2 && y1 <= h - 2;
+        if (wlo && *interior) {
+            float pwmin = pw;
+#pragma unroll
+            for (int m = 1; m < 8; m <<= 1) pwmin = fminf(pwmin, __shfl_xor(pwmin, m));
+            if (pwmin >= fmaxf((float)w, (float)h) * 0x1p-21f * (fmaxf(ex, ey) + ew))
+                *wlo = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(pwmin - 0x1p-18f * ew)));
+        }
     }
     return x1 >= x0 && y1 >= y0;
 }
 
 // 64-bit fetch-min with the benign early-out compare of render.cs:423-426.
 // The plain load can only be stale-high within a frame (values only
-// decrease), so skipping when key >= old never loses a minimum.
-__device__ __forceinline__ void fb_min(unsigned long long* fb, uint32_t pixel, unsigned long long key) {
+// decrease), so skipping when key >= old never loses a minimum. Returns 1
+// when it issued the atomic (the reference's numPointsRendered count,
+// render.cs:428-430).
+__device__ __forceinline__ uint32_t fb_min(unsigned long long* fb, uint32_t pixel, unsigned long long key) {
     const unsigned long long old = fb[pixel];
-    if (key < old) atomicMin(fb + pixel, key);
+    if (key < old) {
+        atomicMin(fb + pixel, key);
+        return 1u;
+    }
+    return 0u;
 }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));

@@ -18,6 +18,10 @@ struct FrameState {
     uint32_t culled;
     unsigned long long points_drawn;
     unsigned long long plane_bytes;
+    uint32_t occluded_batches;          // batch draws skipped by the occlusion test (rect_open)
+    unsigned long long occluded_points; // their points
+    unsigned long long atomics;         // 64-bit framebuffer atomics issued (LAS, LAS2, NODES)
+    unsigned long long pixels_visible;  // pixels the index resolve found covered
 };
 
 void set_error(const char* fmt, ...);
@@ -27,6 +31,16 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     do {                                                                      \
         hipError_t _e = (expr);                                               \
         if (_e != hipSuccess) return ::pcr::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+// PCR_HIP that runs `cleanup` (statements freeing what the caller built so
+// far) before it returns the error.
+#define PCR_HIP_OR(expr, cleanup)                                             \
+    do {                                                                      \
+        hipError_t _e = (expr);                                               \
+        if (_e != hipSuccess) {                                               \
+            cleanup;                                                          \
+            return ::pcr::hip_fail(_e, #expr, __FILE__, __LINE__);            \
+        }                                                                     \
     } while (0)
 #define PCR_CHECK(cond, ...)                                                  \
     do {                                                                      \
@@ -105,14 +119,17 @@ int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pc
                   float first_threshold, bool lod);
 int launch_raster_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int vec, int block,
                           FrameState* st, unsigned long long* fb);
+// occl: the occlusion skip (rect_open) is on
 int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block,
-                        FrameState* st, unsigned long long* fb);
+                        FrameState* st, unsigned long long* fb, bool occl = true);
 int launch_raster_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb);
 int launch_hqs_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb);
 int launch_hqs_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block, FrameState* st,
-                     pcr_fb* fb);
+                     pcr_fb* fb, bool occl = true);
 int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pcr_fb* fb);
-int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, float edl, pcr_fb* fb);
+// st (optional): the frame's state, whose pixels_visible the index resolve counts
+int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, float edl, pcr_fb* fb,
+                   FrameState* st = nullptr);
 int launch_selftest_divide(hipStream_t s, const float* nx, const float* ny, const float* den, uint64_t n,
                            float* out2);
 
@@ -141,9 +158,9 @@ void mv_release(MVBuffers& B);
 int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, uint64_t max_pix,
                    const pcr_render_params& prm, bool lod, const pcr_tune& t, int cu_count);
 int launch_mv_raster_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
-                           int cu_count);
+                           int cu_count, bool occl = true);
 int launch_mv_hqs_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
-                        int cu_count);
+                        int cu_count, bool occl = true);
 
 // layout (pcr_layout.hip)
 int generate_points(pcr_ctx* ctx, const pcr_gen_cfg* cfg, float* xyz, uint32_t* rgba);

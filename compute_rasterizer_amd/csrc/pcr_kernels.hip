@@ -31,6 +31,12 @@ __device__ __forceinline__ void lds_min32(uint32_t* p, uint32_t v) {
 __device__ __forceinline__ void lds_add64(unsigned long long* p, unsigned long long v) {
     __hip_atomic_fetch_add((lds_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// ds_add_u32: the tiled rasters count their framebuffer atomics in an LDS
+// word at the point of issue (a per-thread count would hold a register for
+// the whole point loop).
+__device__ __forceinline__ void lds_add32(uint32_t* p, uint32_t v) {
+    __hip_atomic_fetch_add((lds_u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
@@ -49,6 +55,7 @@ __device__ __forceinline__ void frame_reset(FrameState* st) {
     st->list_count = 0; st->queue_head = 0; st->queue_head2 = 0; st->culled = 0;
     for (int i = 0; i < 5; i++) st->level_count[i] = 0;
     st->points_drawn = 0; st->plane_bytes = 0;
+    st->occluded_batches = 0; st->occluded_points = 0; st->atomics = 0; st->pixels_visible = 0;
 }
 // 16-byte stores (p is hipMalloc-aligned); the last element of an odd-length
 // 64-bit fill, or the last 1-3 of a 32-bit fill, are stored singly.
@@ -222,13 +229,13 @@ int launch_select(hipStream_t s, const pcr_cloud* c, const pcr_view& v, const pc
 template <int LC>
 __device__ __forceinline__ void raster_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi,
                                              uint32_t local, uint32_t np, uint32_t first, const pcr_view& v,
-                                             unsigned long long* __restrict__ fb) {
+                                             unsigned long long* __restrict__ fb, uint32_t& natom) {
     if (local >= np) return;
     float p[3];
     decode<LC>(d, lo, med, hi, p);
     uint32_t pix, depth;
     if (!project(v.transform, p[0], p[1], p[2], v.width, v.height, pix, depth)) return;
-    fb_min(fb, pix, ((unsigned long long)depth << 32) | (first + local));
+    natom += fb_min(fb, pix, ((unsigned long long)depth << 32) | (first + local));
 }
 
 // Four consecutive points per thread per iteration, 128-bit streaming loads of
@@ -237,7 +244,7 @@ template <int BLOCK, int LC>
 __device__ __forceinline__ void raster_vec4(const Decode& d, const uint32_t* __restrict__ lo,
                                             const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                             uint32_t first, uint32_t np, const pcr_view& v,
-                                            unsigned long long* __restrict__ fb) {
+                                            unsigned long long* __restrict__ fb, uint32_t& natom) {
     const uint32_t lead = first & 3u;   // see tile_loop()
     const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
     const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
@@ -248,10 +255,10 @@ __device__ __forceinline__ void raster_vec4(const Decode& d, const uint32_t* __r
         const u32x4 m = LC <= 1 ? ld_stream(med4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 c = LC == 0 ? ld_stream(hi4 + q) : u32x4{0, 0, 0, 0};
         const uint32_t base = q * 4u - lead;
-        raster_point<LC>(d, a.x, m.x, c.x, base + 0, np, first, v, fb);
-        raster_point<LC>(d, a.y, m.y, c.y, base + 1, np, first, v, fb);
-        raster_point<LC>(d, a.z, m.z, c.z, base + 2, np, first, v, fb);
-        raster_point<LC>(d, a.w, m.w, c.w, base + 3, np, first, v, fb);
+        raster_point<LC>(d, a.x, m.x, c.x, base + 0, np, first, v, fb, natom);
+        raster_point<LC>(d, a.y, m.y, c.y, base + 1, np, first, v, fb, natom);
+        raster_point<LC>(d, a.z, m.z, c.z, base + 2, np, first, v, fb, natom);
+        raster_point<LC>(d, a.w, m.w, c.w, base + 3, np, first, v, fb, natom);
     }
 }
 
@@ -261,10 +268,10 @@ template <int BLOCK, int LC>
 __device__ __forceinline__ void raster_strided(const Decode& d, const uint32_t* __restrict__ lo,
                                                const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                                uint32_t first, uint32_t np, const pcr_view& v,
-                                               unsigned long long* __restrict__ fb) {
+                                               unsigned long long* __restrict__ fb, uint32_t& natom) {
     for (uint32_t i = threadIdx.x; i < np; i += BLOCK) {
         const uint32_t idx = first + i;
-        raster_point<LC>(d, lo[idx], LC <= 1 ? med[idx] : 0u, LC == 0 ? hi[idx] : 0u, i, np, first, v, fb);
+        raster_point<LC>(d, lo[idx], LC <= 1 ? med[idx] : 0u, LC == 0 ? hi[idx] : 0u, i, np, first, v, fb, natom);
     }
 }
 
@@ -274,27 +281,32 @@ __global__ __launch_bounds__(BLOCK) void k_raster_batched(const pcr_batch* __res
                                                           const uint32_t* __restrict__ lo,
                                                           const uint32_t* __restrict__ med,
                                                           const uint32_t* __restrict__ hi, pcr_view v,
-                                                          unsigned long long* __restrict__ fb) {
+                                                          unsigned long long* __restrict__ fb,
+                                                          FrameState* __restrict__ st) {
     const uint32_t bi = blockIdx.x;
     const int level = levels[bi];
     if (level == kCulled) return;
     const pcr_batch h = b[bi];
     const Decode d = make_decode(level, h.min, h.max);
+    uint32_t natom = 0;
     if (VEC == 4) {
-        if (level == 0) raster_vec4<BLOCK, 0>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
-        else if (level == 1) raster_vec4<BLOCK, 1>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
-        else raster_vec4<BLOCK, 2>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
+        if (level == 0) raster_vec4<BLOCK, 0>(d, lo, med, hi, h.first_point, h.num_points, v, fb, natom);
+        else if (level == 1) raster_vec4<BLOCK, 1>(d, lo, med, hi, h.first_point, h.num_points, v, fb, natom);
+        else raster_vec4<BLOCK, 2>(d, lo, med, hi, h.first_point, h.num_points, v, fb, natom);
     } else {
-        if (level == 0) raster_strided<BLOCK, 0>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
-        else if (level == 1) raster_strided<BLOCK, 1>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
-        else raster_strided<BLOCK, 2>(d, lo, med, hi, h.first_point, h.num_points, v, fb);
+        if (level == 0) raster_strided<BLOCK, 0>(d, lo, med, hi, h.first_point, h.num_points, v, fb, natom);
+        else if (level == 1) raster_strided<BLOCK, 1>(d, lo, med, hi, h.first_point, h.num_points, v, fb, natom);
+        else raster_strided<BLOCK, 2>(d, lo, med, hi, h.first_point, h.num_points, v, fb, natom);
     }
+    const unsigned long long wa = wave_sum_u64(natom);
+    if (__lane_id() == 0 && wa) atomicAdd(&st->atomics, wa);
 }
 
 template <int BLOCK, int VEC>
-static void raster_batched_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb) {
+static void raster_batched_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, unsigned long long* fb,
+                             FrameState* st) {
     k_raster_batched<BLOCK, VEC><<<c->num_batches, BLOCK, 0, s>>>(c->d_batches, c->d_levels, c->d_lo, c->d_med,
-                                                                  c->d_hi, v, fb);
+                                                                  c->d_hi, v, fb, st);
 }
 
 // --------------------------------------------------------- division check
@@ -334,7 +346,7 @@ template <int LC, bool TILE, bool INTERIOR = false>
 __device__ __forceinline__ void tile_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi, uint32_t local,
                                            uint32_t np, uint32_t first, const Proj& pj,
                                            unsigned long long* __restrict__ fb, unsigned long long* tile, int x0,
-                                           int y0, int tw, int th) {
+                                           int y0, int tw, int th, uint32_t* natom) {
     if (local >= np) return;
     float p[3];
     decode<LC>(d, lo, med, hi, p);
@@ -353,7 +365,7 @@ __device__ __forceinline__ void tile_point(const Decode& d, uint32_t lo, uint32_
     const unsigned long long key = ((unsigned long long)depth << 32) | (first + local);
     const uint32_t ux = (uint32_t)(ix - x0), uy = (uint32_t)(iy - y0);
     if (TILE && ux < (uint32_t)tw && uy < (uint32_t)th) lds_min(tile + uy * (uint32_t)tw + ux, key);
-    else fb_min(fb, (uint32_t)ix + (uint32_t)iy * (uint32_t)pj.w, key);
+    else if (fb_min(fb, (uint32_t)ix + (uint32_t)iy * (uint32_t)pj.w, key)) lds_add32(natom, 1u);
 }
 
 // NT: non-temporal plane loads (a single view streams each plane once);
@@ -369,7 +381,7 @@ __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __res
                                           const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                           uint32_t first, uint32_t np, const Proj& pj,
                                           unsigned long long* __restrict__ fb, unsigned long long* tile, int x0,
-                                          int y0, int tw, int th) {
+                                          int y0, int tw, int th, uint32_t* natom) {
     // A batch may start at any word: quads are read from the aligned word
     // at or below `first`, and the `lead` words in front of the batch get a
     // local index that wraps past np, so the local < np test drops them as
@@ -385,10 +397,14 @@ __device__ __forceinline__ void tile_loop(const Decode& d, const uint32_t* __res
         const u32x4 m = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
         const uint32_t base = q * 4u - lead;
-        tile_point<LC, TILE, INTERIOR>(d, a.x, m.x, c.x, base + 0, np, first, pj, fb, tile, x0, y0, tw, th);
-        tile_point<LC, TILE, INTERIOR>(d, a.y, m.y, c.y, base + 1, np, first, pj, fb, tile, x0, y0, tw, th);
-        tile_point<LC, TILE, INTERIOR>(d, a.z, m.z, c.z, base + 2, np, first, pj, fb, tile, x0, y0, tw, th);
-        tile_point<LC, TILE, INTERIOR>(d, a.w, m.w, c.w, base + 3, np, first, pj, fb, tile, x0, y0, tw, th);
+        tile_point<LC, TILE, INTERIOR>(d, a.x, m.x, c.x, base + 0, np, first, pj, fb, tile, x0, y0, tw, th,
+                                       natom);
+        tile_point<LC, TILE, INTERIOR>(d, a.y, m.y, c.y, base + 1, np, first, pj, fb, tile, x0, y0, tw, th,
+                                       natom);
+        tile_point<LC, TILE, INTERIOR>(d, a.z, m.z, c.z, base + 2, np, first, pj, fb, tile, x0, y0, tw, th,
+                                       natom);
+        tile_point<LC, TILE, INTERIOR>(d, a.w, m.w, c.w, base + 3, np, first, pj, fb, tile, x0, y0, tw, th,
+                                       natom);
     }
 }
 
@@ -399,37 +415,101 @@ __device__ __forceinline__ void tile_dispatch(int level, const Decode& d, const 
                                               const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
                                               const pcr_batch& h, const Proj& pj, unsigned long long* __restrict__ fb,
                                               unsigned long long* tile, int x0, int y0, int tw, int th,
-                                              bool interior = false) {
+                                              uint32_t* natom, bool interior = false) {
     if (level == 0)
-        tile_loop<BLOCK, 0, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th);
+        tile_loop<BLOCK, 0, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th,
+                                      natom);
     else if (level == 1)
-        tile_loop<BLOCK, 1, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th);
+        tile_loop<BLOCK, 1, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th,
+                                      natom);
     else if (TILE && interior)
         tile_loop<BLOCK, 2, true, NT, true>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw,
-                                            th);
+                                            th, natom);
     else
-        tile_loop<BLOCK, 2, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th);
+        tile_loop<BLOCK, 2, TILE, NT>(d, lo, med, hi, h.first_point, h.num_points, pj, fb, tile, x0, y0, tw, th,
+                                      natom);
+}
+
+// Occlusion test of one batch draw against the depths already in the
+// frame's buffer: true when some pixel of the rectangle [x0, x0 + tw) x
+// [y0, y0 + n / tw) may still take a point whose depth is at least wlo.
+// Every point of the batch lands in that rectangle with a depth of at least
+// wlo (screen_rect's `wlo`). Depths only decrease during a frame, so a value
+// read here bounds its pixel's depth from then on (a stale read is only
+// larger), and a batch for which the test returns false cannot change the
+// buffer:
+//   KIND 0 (uint64 keys, depth bits in the high word): each point's key
+//          exceeds the pixel's key;
+//   KIND 1 (hqs pass 1, uint32 depths): each point's depth bits exceed the
+//          pixel's;
+//   KIND 2 (hqs pass 2, final depths): each point fails w <= depth * 1.01f.
+// A cleared pixel is open for kinds 0 and 1 and closed for kind 2 (its depth
+// is NaN, which no point passes). n <= 2^13 and 2 <= tw (the multiply-high row
+// split). Every thread of the workgroup calls it (it ends in a barrier).
+//
+// Most batches are not hidden, and one open pixel settles that: the first
+// wave reads 64 pixels spread over the rectangle, and only when all of them
+// are closed does the workgroup read every pixel. The decision is taken from
+// one block-wide OR, so every wave takes the same branch.
+constexpr int kOcclusionMaxPix = 2048;
+template <int KIND>
+__device__ __forceinline__ bool depth_open(const uint32_t* __restrict__ w32, uint32_t pix, uint32_t wb, float wlo) {
+    // pixel < 2^30 (PCR_MAX_DIM^2), so 2 * pixel + 1 fits 32 bits
+    const uint32_t d = KIND == 0 ? w32[2u * pix + 1u] : w32[pix];
+    return KIND == 2 ? (__uint_as_float(d) * 1.01f >= wlo) : (d >= wb);
+}
+template <int BLOCK, int KIND>
+__device__ __forceinline__ bool rect_open(const void* __restrict__ buf, int width, int x0, int y0, int tw, int n,
+                                          float wlo) {
+    const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
+    const uint32_t* w32 = reinterpret_cast<const uint32_t*>(buf);
+    const uint32_t wb = __float_as_uint(wlo);
+    const auto pixel = [&](uint32_t i) {
+        const uint32_t uy = __umulhi(i, magic), ux = i - uy * (uint32_t)tw;
+        return (uint32_t)x0 + ux + ((uint32_t)y0 + uy) * (uint32_t)width;
+    };
+    int open = 0;
+    if (threadIdx.x < 64) open = depth_open<KIND>(w32, pixel(((uint32_t)n * threadIdx.x) >> 6), wb, wlo);
+    if (__syncthreads_or(open)) return true;
+    constexpr int F = 2;   // two loads in flight per thread (register use stays that of the point loop)
+    for (int i0 = threadIdx.x; i0 < n; i0 += F * BLOCK) {
+        bool o[F];
+#pragma unroll
+        for (int j = 0; j < F; j++) o[j] = depth_open<KIND>(w32, pixel((uint32_t)min(i0 + j * BLOCK, n - 1)), wb, wlo);
+#pragma unroll
+        for (int j = 0; j < F; j++) open |= o[j];
+    }
+    return __syncthreads_or(open) != 0;
 }
 
 // One batch of the tiled raster; `bi` and `level` are workgroup-uniform.
+// Returns the points skipped by the occlusion test (0 if the batch drew).
 template <int BLOCK, int TILE_PIX, bool NT>
-__device__ __forceinline__ void tiled_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
-                                            const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
-                                            const uint32_t* __restrict__ hi, const pcr_view& v,
-                                            unsigned long long* __restrict__ fb, unsigned long long* tile,
-                                            uint32_t part = 0, uint32_t nparts = 1) {
+__device__ __forceinline__ uint32_t tiled_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
+                                                const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
+                                                const uint32_t* __restrict__ hi, const pcr_view& v,
+                                                unsigned long long* __restrict__ fb, unsigned long long* tile,
+                                                uint32_t* natom, bool occl, uint32_t part = 0, uint32_t nparts = 1) {
     pcr_batch h = b[bi];
     if (nparts > 1) {   // points [part * k, (part + 1) * k) of the batch, k = ceil(n / nparts)
         const uint32_t k = (h.num_points + nparts - 1) / nparts, off = min(part * k, h.num_points);
         h.first_point += off;
         h.num_points = min(k, h.num_points - off);
     }
-    const Decode d = make_decode(level, h.min, h.max);
-    const Proj pj = make_proj(v.transform, v.width, v.height);
     int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
     bool interior;
-    bool use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1, &interior);
+    float wlo;
+    bool use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1, &interior, &wlo);
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
+    // a batch behind everything already drawn over its rectangle draws nothing
+    // (wlo > 0 only for interior batches, whose rectangle is not empty); the
+    // test runs before the decode and projection constants exist, so that
+    // its loads do not add to the registers of the point loop
+    if (occl && wlo > 0.0f && tw >= 2 && tw * th <= kOcclusionMaxPix &&
+        !rect_open<BLOCK, 0>(fb, v.width, x0, y0, tw, tw * th, wlo))
+        return h.num_points;
+    const Decode d = make_decode(level, h.min, h.max);
+    const Proj pj = make_proj(v.transform, v.width, v.height);
     // A rectangle one column wide (clamped at the image's left or right edge,
     // where only rounding can put a point inside) draws directly: the tile's
     // row split needs tw >= 2 (ceil(2^32 / tw) must fit 32 bits).
@@ -440,7 +520,7 @@ __device__ __forceinline__ void tiled_batch(uint32_t bi, int level, const pcr_ba
         const int n = tw * th;
         for (int i = threadIdx.x; i < n; i += BLOCK) tile[i] = PCR_FB_CLEAR;
         __syncthreads();
-        tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, pj, fb, tile, x0, y0, tw, th, interior);
+        tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, pj, fb, tile, x0, y0, tw, th, natom, interior);
         __syncthreads();
         // Four touched pixels per thread at a time: their compare loads are
         // all issued before the first is needed, so a thread waits once per
@@ -463,11 +543,15 @@ __device__ __forceinline__ void tiled_batch(uint32_t bi, int level, const pcr_ba
             for (int j = 0; j < F; j++) old[j] = k[j] != PCR_FB_CLEAR ? fb[pix[j]] : 0ull;
 #pragma unroll
             for (int j = 0; j < F; j++)
-                if (k[j] < old[j]) atomicMin(fb + pix[j], k[j]);
+                if (k[j] < old[j]) {
+                    atomicMin(fb + pix[j], k[j]);
+                    lds_add32(natom, 1u);
+                }
         }
     } else {
-        tile_dispatch<BLOCK, false, NT>(level, d, lo, med, hi, h, pj, fb, nullptr, 0, 0, 0, 0);
+        tile_dispatch<BLOCK, false, NT>(level, d, lo, med, hi, h, pj, fb, nullptr, 0, 0, 0, 0, natom);
     }
+    return 0;
 }
 
 // Persistent workgroups take drawn batches from the select stage's compacted
@@ -482,11 +566,13 @@ __global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restr
                                                         const uint32_t* __restrict__ lo,
                                                         const uint32_t* __restrict__ med,
                                                         const uint32_t* __restrict__ hi, pcr_view v,
-                                                        unsigned long long* __restrict__ fb) {
+                                                        unsigned long long* __restrict__ fb, int occl) {
     __shared__ unsigned long long tile[TILE_PIX];
-    __shared__ uint32_t s_item;
+    __shared__ uint32_t s_item, s_occ_batches, s_natom;   // s_natom: framebuffer atomics issued
+    __shared__ unsigned long long s_occ_points;   // batches and points the occlusion test skipped
     const uint32_t count = st->list_count;
     const uint32_t nsplit = min(count, gridDim.x), whole = count - nsplit;
+    if (threadIdx.x == 0) { s_occ_batches = 0; s_occ_points = 0; s_natom = 0; }
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(&st->queue_head, 1u);
         __syncthreads();
@@ -507,13 +593,25 @@ __global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restr
         const uint32_t slot = (uint32_t)(((uint64_t)item * 2654435761ull) % count);
         const uint32_t bi = __builtin_amdgcn_readfirstlane(list[slot]);
         const int level = __builtin_amdgcn_readfirstlane(levels[bi]);
-        tiled_batch<BLOCK, TILE_PIX, true>(bi, level, b, lo, med, hi, v, fb, tile, part, nparts);
+        const uint32_t skipped = tiled_batch<BLOCK, TILE_PIX, true>(bi, level, b, lo, med, hi, v, fb, tile, &s_natom,
+                                                                    occl != 0, part, nparts);
+        if (threadIdx.x == 0 && skipped) {
+            s_occ_points += skipped;
+            s_occ_batches += part == 0;   // a batch drawn as two halves counts once
+        }
         __syncthreads();   // s_item and the tile are reused by the next item
+    }
+    // the loop ends after a barrier that follows every thread's last atomic
+    if (threadIdx.x == 0 && s_natom) atomicAdd(&st->atomics, (unsigned long long)s_natom);
+    if (threadIdx.x == 0 && s_occ_points) {
+        atomicAdd(&st->occluded_batches, s_occ_batches);
+        atomicAdd(&st->occluded_points, s_occ_points);
     }
 }
 
 template <int BLOCK, int TILE_PIX>
-static void tiled_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, unsigned long long* fb) {
+static void tiled_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, unsigned long long* fb,
+                    bool occl) {
     static const int occ = [] {
         int o = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_raster_tiled<BLOCK, TILE_PIX>, BLOCK, 0) != hipSuccess) {
@@ -525,20 +623,20 @@ static void tiled_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameS
     uint64_t grid = (uint64_t)c->ctx->cu_count * (uint64_t)occ;
     if (grid > c->num_batches) grid = c->num_batches;
     k_raster_tiled<BLOCK, TILE_PIX><<<(int)grid, BLOCK, 0, s>>>(c->d_batches, c->d_levels, c->d_list, st, c->d_lo,
-                                                               c->d_med, c->d_hi, v, fb);
+                                                               c->d_med, c->d_hi, v, fb, occl ? 1 : 0);
 }
 
 int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block,
-                        FrameState* st, unsigned long long* fb) {
+                        FrameState* st, unsigned long long* fb, bool occl) {
     if (c->num_batches == 0) return PCR_OK;
     if (block == 128) {
-        if (tile_px == 1024) tiled_t<128, 1024>(s, c, v, st, fb);
-        else if (tile_px == 4096) tiled_t<128, 4096>(s, c, v, st, fb);
-        else tiled_t<128, 2048>(s, c, v, st, fb);
+        if (tile_px == 1024) tiled_t<128, 1024>(s, c, v, st, fb, occl);
+        else if (tile_px == 4096) tiled_t<128, 4096>(s, c, v, st, fb, occl);
+        else tiled_t<128, 2048>(s, c, v, st, fb, occl);
     } else {
-        if (tile_px == 1024) tiled_t<256, 1024>(s, c, v, st, fb);
-        else if (tile_px == 4096) tiled_t<256, 4096>(s, c, v, st, fb);
-        else tiled_t<256, 2048>(s, c, v, st, fb);
+        if (tile_px == 1024) tiled_t<256, 1024>(s, c, v, st, fb, occl);
+        else if (tile_px == 4096) tiled_t<256, 4096>(s, c, v, st, fb, occl);
+        else tiled_t<256, 2048>(s, c, v, st, fb, occl);
     }
     PCR_HIP(hipGetLastError());
     return PCR_OK;
@@ -556,12 +654,13 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster_tiled(const pcr_batch* __re
                                                            const uint32_t* __restrict__ med,
                                                            const uint32_t* __restrict__ hi,
                                                            const pcr_view* __restrict__ views, int nviews,
-                                                           unsigned long long* const* __restrict__ fbs) {
+                                                           unsigned long long* const* __restrict__ fbs, int occl) {
     __shared__ unsigned long long tile[TILE_PIX];
     __shared__ uint32_t s_item, s_nv;
     __shared__ uint8_t s_vid[64], s_lv[64];
     const uint32_t count = st->list_count;
     const uint32_t nsplit = min(count, gridDim.x), whole = count - nsplit;
+    __shared__ uint32_t s_natom;   // counted, not reported per view
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(&st->queue_head, 1u);
         __syncthreads();
@@ -587,14 +686,15 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster_tiled(const pcr_batch* __re
         for (uint32_t k = 0; k < nv; k++) {
             const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
             const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
-            tiled_batch<BLOCK, TILE_PIX, false>(bi, lv, b, lo, med, hi, views[vi], fbs[vi], tile, part, nparts);
+            tiled_batch<BLOCK, TILE_PIX, false>(bi, lv, b, lo, med, hi, views[vi], fbs[vi], tile, &s_natom,
+                                                occl != 0, part, nparts);
             __syncthreads();   // the tile is reused by the next view
         }
     }
 }
 
 template <int BLOCK, int TILE_PIX>
-static void mv_tiled_t(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int cu_count) {
+static void mv_tiled_t(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int cu_count, bool occl) {
     static const int occ = [] {
         int o = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_mv_raster_tiled<BLOCK, TILE_PIX>, BLOCK, 0) !=
@@ -609,19 +709,19 @@ static void mv_tiled_t(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, in
     if (grid == 0) grid = 1;
     k_mv_raster_tiled<BLOCK, TILE_PIX><<<(int)grid, BLOCK, 0, s>>>(c->d_batches, c->num_batches, B.d_levels, B.d_list,
                                                                   B.d_state, c->d_lo, c->d_med, c->d_hi, B.d_views,
-                                                                  nviews, B.d_fbs);
+                                                                  nviews, B.d_fbs, occl ? 1 : 0);
 }
 
 int launch_mv_raster_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
-                           int cu_count) {
+                           int cu_count, bool occl) {
     if (block == 128) {
-        if (tile_px == 1024) mv_tiled_t<128, 1024>(s, c, B, nviews, cu_count);
-        else if (tile_px == 4096) mv_tiled_t<128, 4096>(s, c, B, nviews, cu_count);
-        else mv_tiled_t<128, 2048>(s, c, B, nviews, cu_count);
+        if (tile_px == 1024) mv_tiled_t<128, 1024>(s, c, B, nviews, cu_count, occl);
+        else if (tile_px == 4096) mv_tiled_t<128, 4096>(s, c, B, nviews, cu_count, occl);
+        else mv_tiled_t<128, 2048>(s, c, B, nviews, cu_count, occl);
     } else {
-        if (tile_px == 1024) mv_tiled_t<256, 1024>(s, c, B, nviews, cu_count);
-        else if (tile_px == 4096) mv_tiled_t<256, 4096>(s, c, B, nviews, cu_count);
-        else mv_tiled_t<256, 2048>(s, c, B, nviews, cu_count);
+        if (tile_px == 1024) mv_tiled_t<256, 1024>(s, c, B, nviews, cu_count, occl);
+        else if (tile_px == 4096) mv_tiled_t<256, 4096>(s, c, B, nviews, cu_count, occl);
+        else mv_tiled_t<256, 2048>(s, c, B, nviews, cu_count, occl);
     }
     PCR_HIP(hipGetLastError());
     return PCR_OK;
@@ -760,12 +860,13 @@ __device__ __forceinline__ void hqs_tile_dispatch(int level, const Decode& d, co
                                                 tdepth, tacc, tb);
 }
 
-// Pass 2 keeps 20 B per tile pixel; its tile holds 8 pixels fewer than
-// TILE_PIX so that with the kernels' other shared words four 256-thread
-// workgroups still fit a CU's 160 KB of LDS at TILE_PIX = 2048.
+// Pass 2 keeps 20 B per tile pixel; its tile holds 24 pixels fewer than
+// TILE_PIX so that with the kernels' other shared words (and the block-wide
+// OR of the occlusion test) four 256-thread workgroups still fit a CU's
+// 160 KB of LDS at TILE_PIX = 2048.
 template <int PASS, int TILE_PIX>
 struct HqsTile {
-    static constexpr int kCap = PASS == 1 ? TILE_PIX - 8 : TILE_PIX;
+    static constexpr int kCap = PASS == 1 ? TILE_PIX - 24 : TILE_PIX;
     uint32_t depth[kCap];
     unsigned long long acc[PASS == 1 ? 2 * kCap : 1];
 };
@@ -779,13 +880,15 @@ struct HqsTile {
 // and each touched pixel is flushed once, in coalesced rows.
 constexpr int kHqsMaxBands = 8;
 
+// Returns the points skipped by the occlusion test (0 if the batch drew).
 template <int BLOCK, int TILE_PIX, int PASS, bool NT>
-__device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
+__device__ __forceinline__ uint32_t hqs_tiled_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
                                                 const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
                                                 const uint32_t* __restrict__ hi, const uint32_t* __restrict__ rgba,
                                                 const pcr_view& v, uint32_t* __restrict__ depth,
                                                 uint32_t* __restrict__ acc, uint32_t* tdepth,
-                                                unsigned long long* tacc, uint32_t part = 0, uint32_t nparts = 1) {
+                                                unsigned long long* tacc, bool occl, uint32_t part = 0,
+                                                uint32_t nparts = 1) {
     const pcr_batch h = b[bi];
     pcr_batch hp = h;   // the points this item draws; the tile and band choices follow the whole batch
     if (nparts > 1) {   // points [part * k, (part + 1) * k), k = ceil(n / nparts)
@@ -797,8 +900,15 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
     const Proj pj = make_proj(v.transform, v.width, v.height);
     int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
     bool interior;
-    const bool on_screen = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1, &interior);
+    float wlo;
+    const bool on_screen = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1, &interior,
+                                       &wlo);
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
+    // pass 1: behind every depth already drawn over the rectangle; pass 2:
+    // behind 1.01x every final depth there (rect_open)
+    if (occl && wlo > 0.0f && tw >= 2 && tw * th <= kOcclusionMaxPix &&
+        !rect_open<BLOCK, PASS == 0 ? 1 : 2>(depth, v.width, x0, y0, tw, tw * th, wlo))
+        return hp.num_points;
     constexpr int cap = HqsTile<PASS, TILE_PIX>::kCap;
     // one-column rectangles draw directly, as in tiled_batch (tw >= 2 keeps
     // the row split's ceil(2^32 / tw) within 32 bits)
@@ -812,7 +922,7 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
         const TileBand none{0, 0, 0, 0, 0, 0, true};
         hqs_tile_dispatch<BLOCK, PASS, false, NT>(level, d, lo, med, hi, rgba, hp, pj, depth, acc, nullptr, nullptr,
                                                   none);
-        return;
+        return 0;
     }
     // row = i / tw as a multiply-high (see tiled_batch)
     const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
@@ -849,6 +959,7 @@ __device__ __forceinline__ void hqs_tiled_batch(uint32_t bi, int level, const pc
             }
         }
     }
+    return 0;
 }
 
 // Persistent workgroups over the select stage's list, in the spread order of
@@ -866,12 +977,15 @@ void k_hqs_tiled(const pcr_batch* __restrict__ b,
                  const uint32_t* __restrict__ hi,
                  const uint32_t* __restrict__ rgba, pcr_view v,
                  uint32_t* __restrict__ depth,
-                 uint32_t* __restrict__ acc) {
+                 uint32_t* __restrict__ acc, int occl) {
     __shared__ HqsTile<PASS, TILE_PIX> tile;
     __shared__ uint32_t s_item;
     uint32_t* head = PASS == 0 ? &st->queue_head : &st->queue_head2;
     const uint32_t count = st->list_count;
     const uint32_t nsplit = min(count, gridDim.x), whole = count - nsplit;
+    __shared__ uint32_t s_occ_batches;
+    __shared__ unsigned long long s_occ_points;   // batches and points the occlusion test skipped
+    if (threadIdx.x == 0) { s_occ_batches = 0; s_occ_points = 0; }
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
         __syncthreads();
@@ -883,14 +997,24 @@ void k_hqs_tiled(const pcr_batch* __restrict__ b,
         const uint32_t slot = (uint32_t)(((uint64_t)item * 2654435761ull) % count);
         const uint32_t bi = __builtin_amdgcn_readfirstlane(list[slot]);
         const int level = __builtin_amdgcn_readfirstlane(levels[bi]);
-        hqs_tiled_batch<BLOCK, TILE_PIX, PASS, true>(bi, level, b, lo, med, hi, rgba, v, depth, acc, tile.depth,
-                                                      tile.acc, part, nparts);
+        const uint32_t skipped = hqs_tiled_batch<BLOCK, TILE_PIX, PASS, true>(bi, level, b, lo, med, hi, rgba, v,
+                                                                              depth, acc, tile.depth, tile.acc,
+                                                                              occl != 0, part, nparts);
+        if (threadIdx.x == 0 && skipped) {
+            s_occ_points += skipped;
+            s_occ_batches += part == 0;
+        }
         __syncthreads();   // s_item and the tile are reused by the next item
+    }
+    if (threadIdx.x == 0 && s_occ_points) {
+        atomicAdd(&st->occluded_batches, s_occ_batches);
+        atomicAdd(&st->occluded_points, s_occ_points);
     }
 }
 
 template <int BLOCK, int TILE_PIX, int PASS>
-static void hqs_tiled_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb) {
+static void hqs_tiled_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb,
+                        bool occl) {
     static const int occ = [] {
         int o = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_hqs_tiled<BLOCK, TILE_PIX, PASS>, BLOCK, 0) !=
@@ -904,26 +1028,27 @@ static void hqs_tiled_t(hipStream_t s, const pcr_cloud* c, const pcr_view& v, Fr
     if (grid > c->num_batches) grid = c->num_batches;
     k_hqs_tiled<BLOCK, TILE_PIX, PASS><<<(int)grid, BLOCK, 0, s>>>(c->d_batches, c->d_levels, c->d_list, st, c->d_lo,
                                                                    c->d_med, c->d_hi, c->d_rgba, v, fb->d_depth,
-                                                                   fb->d_acc);
+                                                                   fb->d_acc, occl ? 1 : 0);
 }
 
 template <int BLOCK, int TILE_PIX>
-static void hqs_tiled_passes(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb) {
-    hqs_tiled_t<BLOCK, TILE_PIX, 0>(s, c, v, st, fb);
-    hqs_tiled_t<BLOCK, TILE_PIX, 1>(s, c, v, st, fb);
+static void hqs_tiled_passes(hipStream_t s, const pcr_cloud* c, const pcr_view& v, FrameState* st, pcr_fb* fb,
+                             bool occl) {
+    hqs_tiled_t<BLOCK, TILE_PIX, 0>(s, c, v, st, fb, occl);
+    hqs_tiled_t<BLOCK, TILE_PIX, 1>(s, c, v, st, fb, occl);
 }
 
 int launch_hqs_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int tile_px, int block, FrameState* st,
-                     pcr_fb* fb) {
+                     pcr_fb* fb, bool occl) {
     if (c->num_batches == 0) return PCR_OK;
     if (block == 128) {
-        if (tile_px == 1024) hqs_tiled_passes<128, 1024>(s, c, v, st, fb);
-        else if (tile_px == 4096) hqs_tiled_passes<128, 4096>(s, c, v, st, fb);
-        else hqs_tiled_passes<128, 2048>(s, c, v, st, fb);
+        if (tile_px == 1024) hqs_tiled_passes<128, 1024>(s, c, v, st, fb, occl);
+        else if (tile_px == 4096) hqs_tiled_passes<128, 4096>(s, c, v, st, fb, occl);
+        else hqs_tiled_passes<128, 2048>(s, c, v, st, fb, occl);
     } else {
-        if (tile_px == 1024) hqs_tiled_passes<256, 1024>(s, c, v, st, fb);
-        else if (tile_px == 4096) hqs_tiled_passes<256, 4096>(s, c, v, st, fb);
-        else hqs_tiled_passes<256, 2048>(s, c, v, st, fb);
+        if (tile_px == 1024) hqs_tiled_passes<256, 1024>(s, c, v, st, fb, occl);
+        else if (tile_px == 4096) hqs_tiled_passes<256, 4096>(s, c, v, st, fb, occl);
+        else hqs_tiled_passes<256, 2048>(s, c, v, st, fb, occl);
     }
     PCR_HIP(hipGetLastError());
     return PCR_OK;
@@ -942,7 +1067,7 @@ __global__ __launch_bounds__(BLOCK) void k_mv_hqs_tiled(const pcr_batch* __restr
                                                         const uint32_t* __restrict__ rgba,
                                                         const pcr_view* __restrict__ views, int nviews,
                                                         uint32_t* const* __restrict__ depths,
-                                                        uint32_t* const* __restrict__ accs) {
+                                                        uint32_t* const* __restrict__ accs, int occl) {
     __shared__ HqsTile<PASS, TILE_PIX> tile;
     __shared__ uint32_t s_item, s_nv;
     __shared__ uint8_t s_vid[64], s_lv[64];
@@ -975,14 +1100,15 @@ __global__ __launch_bounds__(BLOCK) void k_mv_hqs_tiled(const pcr_batch* __restr
             const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
             const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
             hqs_tiled_batch<BLOCK, TILE_PIX, PASS, false>(bi, lv, b, lo, med, hi, rgba, views[vi], depths[vi],
-                                                           accs[vi], tile.depth, tile.acc, part, nparts);
+                                                           accs[vi], tile.depth, tile.acc, occl != 0, part, nparts);
             __syncthreads();   // the tile is reused by the next view
         }
     }
 }
 
 template <int BLOCK, int TILE_PIX, int PASS>
-static void mv_hqs_tiled_t(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int cu_count) {
+static void mv_hqs_tiled_t(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int cu_count,
+                           bool occl) {
     static const int occ = [] {
         int o = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_mv_hqs_tiled<BLOCK, TILE_PIX, PASS>, BLOCK, 0) !=
@@ -998,25 +1124,26 @@ static void mv_hqs_tiled_t(hipStream_t s, const pcr_cloud* c, const MVBuffers& B
     k_mv_hqs_tiled<BLOCK, TILE_PIX, PASS><<<(int)grid, BLOCK, 0, s>>>(c->d_batches, c->num_batches, B.d_levels,
                                                                       B.d_list, B.d_state, c->d_lo, c->d_med, c->d_hi,
                                                                       c->d_rgba, B.d_views, nviews, B.d_depths,
-                                                                      B.d_accs);
+                                                                      B.d_accs, occl ? 1 : 0);
 }
 
 template <int BLOCK, int TILE_PIX>
-static void mv_hqs_tiled_passes(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int cu_count) {
-    mv_hqs_tiled_t<BLOCK, TILE_PIX, 0>(s, c, B, nviews, cu_count);
-    mv_hqs_tiled_t<BLOCK, TILE_PIX, 1>(s, c, B, nviews, cu_count);
+static void mv_hqs_tiled_passes(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int cu_count,
+                                bool occl) {
+    mv_hqs_tiled_t<BLOCK, TILE_PIX, 0>(s, c, B, nviews, cu_count, occl);
+    mv_hqs_tiled_t<BLOCK, TILE_PIX, 1>(s, c, B, nviews, cu_count, occl);
 }
 
 int launch_mv_hqs_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
-                        int cu_count) {
+                        int cu_count, bool occl) {
     if (block == 128) {
-        if (tile_px == 1024) mv_hqs_tiled_passes<128, 1024>(s, c, B, nviews, cu_count);
-        else if (tile_px == 4096) mv_hqs_tiled_passes<128, 4096>(s, c, B, nviews, cu_count);
-        else mv_hqs_tiled_passes<128, 2048>(s, c, B, nviews, cu_count);
+        if (tile_px == 1024) mv_hqs_tiled_passes<128, 1024>(s, c, B, nviews, cu_count, occl);
+        else if (tile_px == 4096) mv_hqs_tiled_passes<128, 4096>(s, c, B, nviews, cu_count, occl);
+        else mv_hqs_tiled_passes<128, 2048>(s, c, B, nviews, cu_count, occl);
     } else {
-        if (tile_px == 1024) mv_hqs_tiled_passes<256, 1024>(s, c, B, nviews, cu_count);
-        else if (tile_px == 4096) mv_hqs_tiled_passes<256, 4096>(s, c, B, nviews, cu_count);
-        else mv_hqs_tiled_passes<256, 2048>(s, c, B, nviews, cu_count);
+        if (tile_px == 1024) mv_hqs_tiled_passes<256, 1024>(s, c, B, nviews, cu_count, occl);
+        else if (tile_px == 4096) mv_hqs_tiled_passes<256, 4096>(s, c, B, nviews, cu_count, occl);
+        else mv_hqs_tiled_passes<256, 2048>(s, c, B, nviews, cu_count, occl);
     }
     PCR_HIP(hipGetLastError());
     return PCR_OK;
@@ -1024,21 +1151,20 @@ int launch_mv_hqs_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, i
 
 int launch_raster_batched(hipStream_t s, const pcr_cloud* c, const pcr_view& v, int vec, int block,
                           FrameState* st, unsigned long long* fb) {
-    (void)st;
     if (c->num_batches == 0) return PCR_OK;
     if (vec == 4) {
         switch (block) {
-            case 64: raster_batched_t<64, 4>(s, c, v, fb); break;
-            case 128: raster_batched_t<128, 4>(s, c, v, fb); break;
-            case 512: raster_batched_t<512, 4>(s, c, v, fb); break;
-            default: raster_batched_t<256, 4>(s, c, v, fb); break;
+            case 64: raster_batched_t<64, 4>(s, c, v, fb, st); break;
+            case 128: raster_batched_t<128, 4>(s, c, v, fb, st); break;
+            case 512: raster_batched_t<512, 4>(s, c, v, fb, st); break;
+            default: raster_batched_t<256, 4>(s, c, v, fb, st); break;
         }
     } else {
         switch (block) {
-            case 64: raster_batched_t<64, 1>(s, c, v, fb); break;
-            case 256: raster_batched_t<256, 1>(s, c, v, fb); break;
-            case 512: raster_batched_t<512, 1>(s, c, v, fb); break;
-            default: raster_batched_t<128, 1>(s, c, v, fb); break;
+            case 64: raster_batched_t<64, 1>(s, c, v, fb, st); break;
+            case 256: raster_batched_t<256, 1>(s, c, v, fb, st); break;
+            case 512: raster_batched_t<512, 1>(s, c, v, fb, st); break;
+            default: raster_batched_t<128, 1>(s, c, v, fb, st); break;
         }
     }
     PCR_HIP(hipGetLastError());
@@ -1170,22 +1296,29 @@ int launch_hqs_points16(hipStream_t s, const pcr_cloud* c, const pcr_view& v, pc
 // resolves to the background instead of reading out of bounds. strength > 0
 // adds the eye-dome lighting shade (resolve_index_pixel). 256 threads per
 // workgroup, one 16x16 tile per iteration (tile_pixel).
+// `visible` (optional) counts the pixels holding a point, as the reference's
+// resolve counts numPointsVisible (resolve.cs:143-145): one atomic per wave.
 __global__ __launch_bounds__(256) void k_resolve_index(const unsigned long long* __restrict__ fb, int w, int h,
                                                        const uint32_t* __restrict__ rgba, uint64_t npoints,
-                                                       uint32_t bg, float strength, uint32_t* __restrict__ out) {
+                                                       uint32_t bg, float strength, uint32_t* __restrict__ out,
+                                                       unsigned long long* __restrict__ visible) {
     const uint32_t tiles_x = ((uint32_t)w + 15u) >> 4, tiles = tiles_x * (((uint32_t)h + 15u) >> 4);
+    uint32_t covered = 0;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         uint32_t x, y;
         tile_pixel(tile, tiles_x, x, y);
         if (x >= (uint32_t)w || y >= (uint32_t)h) continue;
         const uint64_t i = (uint64_t)y * (uint32_t)w + x;
-        if (strength > 0.0f) {
-            out[i] = resolve_index_pixel(fb, w, h, (int)x, (int)y, rgba, npoints, bg, strength);
-        } else {
-            const unsigned long long k = fb[i];
-            const uint32_t id = (uint32_t)k;
-            out[i] = (k == PCR_FB_CLEAR || id >= npoints) ? bg : rgba[id];
-        }
+        const unsigned long long k = fb[i];
+        const uint32_t id = (uint32_t)k;
+        const bool point = k != PCR_FB_CLEAR && id < npoints;
+        covered += point;
+        if (strength > 0.0f) out[i] = resolve_index_pixel(fb, w, h, (int)x, (int)y, rgba, npoints, bg, strength);
+        else out[i] = point ? rgba[id] : bg;
+    }
+    if (visible) {
+        const unsigned long long n = wave_sum_u64(covered);
+        if (__lane_id() == 0 && n) atomicAdd(visible, n);
     }
 }
 // Colour payload: the key's low 32 bits are the RGBA (2021 resolve.cs:45-57).
@@ -1218,7 +1351,8 @@ __global__ __launch_bounds__(256) void k_resolve_hqs_window(const uint32_t* __re
     }
 }
 
-int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, float edl, pcr_fb* fb) {
+int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int window, float edl, pcr_fb* fb,
+                   FrameState* st) {
     const uint64_t npix = (uint64_t)fb->width * fb->height;
     const int g = grid_for(npix, 256);
     if (mode == PCR_MODE_NODES_HQS) {
@@ -1230,7 +1364,7 @@ int launch_resolve(hipStream_t s, const pcr_cloud* c, int mode, uint32_t bg, int
         k_resolve_color<<<g, 256, 0, s>>>(fb->d_keys, npix, bg, fb->d_rgba);
     } else {
         k_resolve_index<<<g, 256, 0, s>>>(fb->d_keys, fb->width, fb->height, c->d_rgba, c->num_points, bg, edl,
-                                          fb->d_rgba);
+                                          fb->d_rgba, st ? &st->pixels_visible : nullptr);
     }
     PCR_HIP(hipGetLastError());
     return PCR_OK;

@@ -406,7 +406,7 @@ int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nv
                                              B.d_state, c->d_lo, c->d_med, c->d_hi, c->d_rgba, B.d_views, nviews, \
                                              B.d_depths, B.d_accs, chunk_pts, cpb)
         if (t.tile_px) {
-            int rc = launch_mv_hqs_tiled(s, c, B, nviews, t.tile_px, t.block_size, cu_count);
+            int rc = launch_mv_hqs_tiled(s, c, B, nviews, t.tile_px, t.block_size, cu_count, prm.occlusion_skip == 0);
             if (rc) return rc;
         } else {
             for (int pass = 0; pass < 2; pass++) {
@@ -427,7 +427,7 @@ int mv_enqueue_raw(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nv
                                           B.d_state, c->d_lo, c->d_med, c->d_hi, B.d_views, nviews, B.d_fbs,    \
                                           chunk_pts, cpb)
     if (t.tile_px) {
-        int rc = launch_mv_raster_tiled(s, c, B, nviews, t.tile_px, t.block_size, cu_count);
+        int rc = launch_mv_raster_tiled(s, c, B, nviews, t.tile_px, t.block_size, cu_count, prm.occlusion_skip == 0);
         if (rc) return rc;
     } else if (t.block_size == 64) MV_RASTER(64);
     else if (t.block_size == 128) MV_RASTER(128);
@@ -527,17 +527,18 @@ int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, 
         mv->max_pix = std::max<uint64_t>(mv->max_pix, (uint64_t)views[v].width * views[v].height);
     }
     int rc0 = mv_alloc(mv->B, nviews, cloud->num_batches);
-    if (rc0) { delete mv; return rc0; }
+    if (rc0) { mv_release(mv->B); delete mv; return rc0; }
     hipStream_t s = ctx->stream;
-    PCR_HIP(hipMemcpyAsync(mv->B.d_views, views, sizeof(pcr_view) * nviews, hipMemcpyHostToDevice, s));
-    PCR_HIP(hipMemcpyAsync(mv->B.d_fbs, hf.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
-    PCR_HIP(hipMemcpyAsync(mv->B.d_depths, hd.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
-    PCR_HIP(hipMemcpyAsync(mv->B.d_accs, ha.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
-    PCR_HIP(hipMemcpyAsync(mv->B.d_out, ho.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s));
-    PCR_HIP(hipMemcpyAsync(mv->B.d_lod, hl.data(), sizeof(float) * nviews, hipMemcpyHostToDevice, s));
-    PCR_HIP(hipStreamSynchronize(s));
+    const auto drop = [mv, s] { (void)hipStreamSynchronize(s); mv_free(mv); delete mv; };
+    PCR_HIP_OR(hipMemcpyAsync(mv->B.d_views, views, sizeof(pcr_view) * nviews, hipMemcpyHostToDevice, s), drop());
+    PCR_HIP_OR(hipMemcpyAsync(mv->B.d_fbs, hf.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s), drop());
+    PCR_HIP_OR(hipMemcpyAsync(mv->B.d_depths, hd.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s), drop());
+    PCR_HIP_OR(hipMemcpyAsync(mv->B.d_accs, ha.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s), drop());
+    PCR_HIP_OR(hipMemcpyAsync(mv->B.d_out, ho.data(), sizeof(void*) * nviews, hipMemcpyHostToDevice, s), drop());
+    PCR_HIP_OR(hipMemcpyAsync(mv->B.d_lod, hl.data(), sizeof(float) * nviews, hipMemcpyHostToDevice, s), drop());
+    PCR_HIP_OR(hipStreamSynchronize(s), drop());
     if (use_graph) {
-        PCR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        PCR_HIP_OR(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), drop());
         int rc = mv_enqueue(mv, s);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(s, &g);
@@ -549,7 +550,7 @@ int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, 
             return rc;
         }
         mv->graph = g;
-        PCR_HIP(hipGraphInstantiate(&mv->exec, g, nullptr, nullptr, 0));
+        PCR_HIP_OR(hipGraphInstantiate(&mv->exec, g, nullptr, nullptr, 0), (mv->exec = nullptr, drop()));
     }
     *out = mv;
     return PCR_OK;
@@ -604,25 +605,30 @@ int pcr_autotune(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nvie
     }
     for (int bsz : {128, 256})
         for (int tile : {1024, 2048, 4096}) space.push_back(pcr_tune{bsz, 40, 0, tile});
-    hipEvent_t e0, e1;
-    PCR_HIP(hipEventCreate(&e0));
-    PCR_HIP(hipEventCreate(&e1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const auto drop_events = [&] {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    PCR_HIP_OR(hipEventCreate(&e0), drop_events());
+    PCR_HIP_OR(hipEventCreate(&e1), drop_events());
     float best_t = 3.0e38f;
     pcr_tune bt = space[0];
     for (const pcr_tune& cand : space) {
         pcr_tune t = cand;
         pcr_multiview* mv = nullptr;
         int rc = pcr_multiview_create(ctx, cloud, views, nviews, fbs, params, &t, 1, &mv);
-        if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+        if (rc) { drop_events(); return rc; }
+        const auto drop = [&] { pcr_multiview_destroy(mv); drop_events(); };
         std::vector<float> ts;
         for (int it = 0; it < iters + 1; it++) {
-            PCR_HIP(hipEventRecord(e0, ctx->stream));
+            PCR_HIP_OR(hipEventRecord(e0, ctx->stream), drop());
             rc = pcr_multiview_launch(mv);
-            if (rc) { pcr_multiview_destroy(mv); return rc; }
-            PCR_HIP(hipEventRecord(e1, ctx->stream));
-            PCR_HIP(hipEventSynchronize(e1));
+            if (rc) { drop(); return rc; }
+            PCR_HIP_OR(hipEventRecord(e1, ctx->stream), drop());
+            PCR_HIP_OR(hipEventSynchronize(e1), drop());
             float ms = 0;
-            PCR_HIP(hipEventElapsedTime(&ms, e0, e1));
+            PCR_HIP_OR(hipEventElapsedTime(&ms, e0, e1), drop());
             if (it > 0) ts.push_back(ms);   // the first launch warms up
         }
         pcr_multiview_destroy(mv);
@@ -630,8 +636,7 @@ int pcr_autotune(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, int nvie
         const float med = ts[ts.size() / 2];
         if (med < best_t) { best_t = med; bt = t; }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    drop_events();
     *best = bt;
     *best_ms = best_t;
     return PCR_OK;

@@ -13,7 +13,9 @@
 //     arrive while others run are launched on another stream and overlap;
 //   - completion: the resolved image is copied to a pinned host arena on the
 //     same stream and an event marks the request done; poll/wait copy it out
-//     and release its memory.
+//     and release its memory. The pinned arena holds 4 B per pixel and is
+//     sized min(budget / 3, kMaxPinnedBytes); a request is admitted when its
+//     framebuffer fits the device arena and its image the pinned one.
 // Each request's image depends only on its cloud and view (min is order
 // independent), so it is the same alone and in any mix.
 #include <chrono>
@@ -72,6 +74,7 @@ struct Req {
     pcr_request r{};
     int state = kPending;
     uint64_t off = 0, bytes = 0, npix = 0;
+    uint64_t hoff = 0, hbytes = 0;           // the image's range of the pinned arena
     double t_submit = 0, t_done = 0;
 };
 
@@ -88,6 +91,9 @@ struct Slot {
 };
 
 static uint64_t fb_bytes(const pcr_view& v) { return 12ull * (uint64_t)v.width * (uint64_t)v.height; }
+// Pinned host memory for finished images is capped: page-locked memory is
+// taken from the host's RAM for the server's lifetime.
+constexpr uint64_t kMaxPinnedBytes = 1ull << 30;
 
 }  // namespace pcr
 
@@ -97,8 +103,9 @@ struct pcr_server {
     pcr_ctx* ctx = nullptr;
     int max_batch = 16;
     unsigned char* d_arena = nullptr;
-    unsigned char* h_arena = nullptr;       // pinned, one third of the device arena
-    Arena arena;
+    unsigned char* h_arena = nullptr;       // pinned result images (harena)
+    Arena arena;                            // device framebuffers, 12 B per pixel
+    Arena harena;                           // pinned images, 4 B per pixel
     std::vector<pcr_cloud*> clouds;
     uint64_t max_nb = 0;
     std::mutex mu;
@@ -161,7 +168,7 @@ static int launch_group(pcr_server* srv, Slot& sl, const std::vector<uint64_t>& 
     if (rc) return rc;
     for (int k = 0; k < n; k++) {
         const Req& q = srv->reqs[group[k]];
-        PCR_HIP(hipMemcpyAsync(srv->h_arena + q.off / 3, sl.h_out[k], 4 * q.npix, hipMemcpyDeviceToHost, s));
+        PCR_HIP(hipMemcpyAsync(srv->h_arena + q.hoff, sl.h_out[k], 4 * q.npix, hipMemcpyDeviceToHost, s));
     }
     PCR_HIP(hipEventRecord(sl.done, s));
     sl.tickets = group;
@@ -186,6 +193,7 @@ static void worker_main(pcr_server* srv) {
                 r.t_done = t;
                 if (r.state == kCancelled) {
                     srv->arena.release(r.off, r.bytes);
+                    srv->harena.release(r.hoff, r.hbytes);
                     srv->reqs.erase(tk);
                 } else {
                     r.state = kDone;
@@ -209,6 +217,10 @@ static void worker_main(pcr_server* srv) {
                 Req& q = srv->reqs[srv->pending[i]];
                 if (q.r.cloud_id != head.r.cloud_id || q.r.mode != head.r.mode) continue;
                 if (!srv->arena.alloc(q.bytes, q.off)) break;   // wait for memory, keep FIFO order
+                if (!srv->harena.alloc(q.hbytes, q.hoff)) {
+                    srv->arena.release(q.off, q.bytes);
+                    break;
+                }
                 group.push_back(srv->pending[i]);
                 taken.push_back(i);
             }
@@ -244,8 +256,10 @@ int pcr_server_create(pcr_ctx* ctx, uint64_t budget, int max_batch, int num_stre
     srv->ctx = ctx;
     srv->max_batch = max_batch;
     srv->arena.init(budget);
+    srv->harena.gran = 256;
+    srv->harena.init(budget / 3 < kMaxPinnedBytes ? budget / 3 : kMaxPinnedBytes);
     if (hipMalloc(&srv->d_arena, srv->arena.size) != hipSuccess ||
-        hipHostMalloc(&srv->h_arena, srv->arena.size / 3) != hipSuccess) {
+        hipHostMalloc(&srv->h_arena, srv->harena.size) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(srv->d_arena);
         delete srv;
@@ -254,8 +268,13 @@ int pcr_server_create(pcr_ctx* ctx, uint64_t budget, int max_batch, int num_stre
     }
     srv->slots.resize(num_streams);
     for (Slot& sl : srv->slots) {
-        PCR_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-        PCR_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            pcr_server_destroy(srv);
+            set_error("server_create: cannot create the server's streams and events");
+            return PCR_E_HIP;
+        }
     }
     srv->worker = std::thread(worker_main, srv);
     *out = srv;
@@ -290,17 +309,22 @@ int pcr_server_submit(pcr_server* srv, const pcr_request* req, uint64_t* ticket)
     PCR_CHECK(req->view.width > 0 && req->view.height > 0 && req->view.width <= 16384 && req->view.height <= 16384,
               "server_submit: bad view size %dx%d", req->view.width, req->view.height);
     const uint64_t bytes = srv->arena.round(fb_bytes(req->view));
-    if (bytes > srv->arena.size) {
+    const uint64_t npix = (uint64_t)req->view.width * req->view.height;
+    const uint64_t hbytes = srv->harena.round(4 * npix);
+    if (bytes > srv->arena.size || hbytes > srv->harena.size) {
         srv->st.refused++;
-        set_error("server_submit: a %dx%d framebuffer (%llu B) exceeds the %llu-byte budget", req->view.width,
-                  req->view.height, (unsigned long long)bytes, (unsigned long long)srv->arena.size);
+        set_error("server_submit: a %dx%d framebuffer (%llu B, image %llu B) exceeds the %llu-byte budget "
+                  "(pinned images %llu B)", req->view.width, req->view.height, (unsigned long long)bytes,
+                  (unsigned long long)hbytes, (unsigned long long)srv->arena.size,
+                  (unsigned long long)srv->harena.size);
         return PCR_E_BUDGET;
     }
     if (srv->worker_error) { set_error("server_submit: the server stopped after a HIP error"); return PCR_E_HIP; }
     Req r;
     r.r = *req;
     r.bytes = bytes;
-    r.npix = (uint64_t)req->view.width * req->view.height;
+    r.npix = npix;
+    r.hbytes = hbytes;
     r.t_submit = now_ms();
     const uint64_t t = srv->next_ticket++;
     srv->reqs.emplace(t, r);
@@ -324,12 +348,13 @@ static int take_locked(pcr_server* srv, std::unique_lock<std::mutex>& lk, uint64
     if (r.state != kDone) return PCR_E_NOT_READY;
     if (latency_ms) *latency_ms = r.t_done - r.t_submit;
     r.state = kTaking;
-    const unsigned char* src = srv->h_arena + r.off / 3;
-    const uint64_t npix = r.npix, off = r.off, bytes = r.bytes;
+    const unsigned char* src = srv->h_arena + r.hoff;
+    const uint64_t npix = r.npix, off = r.off, bytes = r.bytes, hoff = r.hoff, hbytes = r.hbytes;
     lk.unlock();
     if (rgba) memcpy(rgba, src, 4 * npix);
     lk.lock();
     srv->arena.release(off, bytes);
+    srv->harena.release(hoff, hbytes);
     srv->reqs.erase(ticket);
     srv->work_cv.notify_one();
     return PCR_OK;
@@ -364,7 +389,7 @@ int pcr_server_wait_pinned(pcr_server* srv, uint64_t ticket, const uint32_t** rg
                   (unsigned long long)ticket);
         if (r.state == kDone) {
             r.state = kHeld;
-            *rgba_pinned = reinterpret_cast<const uint32_t*>(srv->h_arena + r.off / 3);
+            *rgba_pinned = reinterpret_cast<const uint32_t*>(srv->h_arena + r.hoff);
             if (latency_ms) *latency_ms = r.t_done - r.t_submit;
             return PCR_OK;
         }
@@ -383,6 +408,7 @@ int pcr_server_release(pcr_server* srv, uint64_t ticket) {
     PCR_CHECK(it != srv->reqs.end() && it->second.state == kHeld, "server_release: ticket %llu is not held",
               (unsigned long long)ticket);
     srv->arena.release(it->second.off, it->second.bytes);
+    srv->harena.release(it->second.hoff, it->second.hbytes);
     srv->reqs.erase(it);
     srv->work_cv.notify_one();
     return PCR_OK;
@@ -402,6 +428,7 @@ int pcr_server_cancel(pcr_server* srv, uint64_t ticket) {
         r.state = kCancelled;            // released when its launch retires
     } else if (r.state == kDone || r.state == kHeld) {
         srv->arena.release(r.off, r.bytes);
+        srv->harena.release(r.hoff, r.hbytes);
         srv->reqs.erase(it);
     }   // kTaking: another call is copying the finished image out; nothing to cancel
     return PCR_OK;
@@ -425,12 +452,12 @@ int pcr_server_destroy(pcr_server* srv) {
     }
     if (srv->worker.joinable()) srv->worker.join();
     for (Slot& sl : srv->slots) {
-        (void)hipStreamSynchronize(sl.stream);
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         mv_release(sl.B);
         (void)hipHostFree(sl.h_views); (void)hipHostFree(sl.h_lod); (void)hipHostFree(sl.h_fbs);
         (void)hipHostFree(sl.h_out);
-        (void)hipEventDestroy(sl.done);
-        (void)hipStreamDestroy(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     (void)hipFree(srv->d_arena);
     (void)hipHostFree(srv->h_arena);

@@ -88,6 +88,8 @@ extern "C" int pcr_render_static(const pcr_batch* batches_dev, uint32_t nbatches
               "render_static: bad lod_rule %d", params->lod_rule);
     PCR_CHECK(params->tile_px == 0 || params->tile_px == 1024 || params->tile_px == 2048 || params->tile_px == 4096,
               "render_static: tile_px %d not in {0,1024,2048,4096}", params->tile_px);
+    PCR_CHECK(params->occlusion_skip == 0 || params->occlusion_skip == -1,
+              "render_static: occlusion_skip %d not 0 or -1", params->occlusion_skip);
     const int block = params->block_size;
     PCR_CHECK(block == 0 || block == 64 || block == 128 || block == 256 || block == 512,
               "render_static: block_size %d not in {64,128,256,512}", block);
@@ -133,6 +135,7 @@ extern "C" int pcr_render_static(const pcr_batch* batches_dev, uint32_t nbatches
         f0 = FillRegion{reinterpret_cast<uint32_t*>(keys_dev), 2ull * view->width * view->height, 0xFFFFFFFFu};
     if ((rc = launch_select(s, &c, *view, *params, st, st + 1, f0, f1, lod ? 80.0f : 100.0f, lod))) return rc;
     if (mode == PCR_MODE_LAS) return launch_raster_batched(s, &c, *view, 1, block ? block : 128, st, keys);
-    return params->tile_px ? launch_raster_tiled(s, &c, *view, params->tile_px, block, st, keys)
+    return params->tile_px ? launch_raster_tiled(s, &c, *view, params->tile_px, block, st, keys,
+                                                 params->occlusion_skip == 0)
                            : launch_raster_batched(s, &c, *view, 4, block ? block : 256, st, keys);
 }

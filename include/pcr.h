@@ -134,8 +134,23 @@ typedef struct pcr_render_params {
                                    shading that compute_loop_las/resolve.cs:
                                    148-191 carries switched off (its constant is
                                    0.0005); 0 = off                              */
+    int32_t  occlusion_skip;    /* tiled LAS2/NODES/HQS/NODES_HQS raster: 0 = on
+                                   (default), -1 = off. A batch whose nearest
+                                   possible depth lies behind every depth already
+                                   drawn over its screen rectangle is skipped;
+                                   the frame is the same bit for bit either way */
+    int32_t  debug_counters;    /* 1: the index resolve also counts the covered
+                                   pixels (pcr_stats.pixels_visible), as the
+                                   reference's resolve does with debug.enabled
+                                   (resolve.cs:143-145); 0 = off              */
 } pcr_render_params;
 
+/* Statistics of the last frame rendered on a context; the counts follow the
+ * reference's debug counters (compute_loop_las/render.cs:73-81, read back at
+ * compute_loop_las.h:260-277): #nodes processed = batches_total, #nodes
+ * rendered = batches_total - batches_culled, #points processed =
+ * points_drawn, #points rendered = atomics_issued, #points visible =
+ * pixels_visible. */
 typedef struct pcr_stats {
     uint64_t batches_total;
     uint64_t batches_culled;       /* frustum-culled or LOD-skipped               */
@@ -146,6 +161,17 @@ typedef struct pcr_stats {
     float    ms_raster;
     float    ms_resolve;
     float    ms_clear;
+    uint64_t batches_occluded;     /* tiled LAS2/NODES/HQS/NODES_HQS draws skipped
+                                      because every pixel of the batch's screen
+                                      rectangle already held a nearer point (hqs:
+                                      both passes, a batch per pass)            */
+    uint64_t points_occluded;      /* the points of those batches                 */
+    uint64_t atomics_issued;       /* 64-bit framebuffer atomicMins issued (LAS,
+                                      LAS2, NODES): those whose compare load found
+                                      a larger key                                */
+    uint64_t pixels_visible;       /* pixels holding a point after the index
+                                      resolve (LAS, LAS2, NODES) with
+                                      params.debug_counters = 1, else 0          */
 } pcr_stats;
 
 typedef struct pcr_ctx   pcr_ctx;

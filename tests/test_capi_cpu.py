@@ -35,10 +35,34 @@ def test_python_signatures_cover_header():
     assert set(declared_symbols()) <= set(capi.SIGNATURES)
 
 
-def test_struct_sizes_match_header():
+# ctypes mirror -> C type in include/pcr.h
+MIRRORS = {"Batch": "pcr_batch", "View": "pcr_view", "RenderParams": "pcr_render_params", "Stats": "pcr_stats",
+           "GenCfg": "pcr_gen_cfg", "BuildOpts": "pcr_build_opts", "CloudInfo": "pcr_cloud_info",
+           "LasInfo": "pcr_las_info", "LasFile": "pcr_las_file", "PotreeInfo": "pcr_potree_info",
+           "OctreeOpts": "pcr_octree_opts", "Tune": "pcr_tune", "Request": "pcr_request",
+           "ServerStats": "pcr_server_stats"}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets gcc gives the header's struct."""
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pcr.h"', "int main(void) {"]
+    for py, c in MIRRORS.items():
+        lines.append(f'printf("{py} __sizeof__ %zu\\n", sizeof({c}));')
+        for name, _ in getattr(capi, py)._fields_:
+            lines.append(f'printf("{py} {name} %zu\\n", offsetof({c}, {name}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for py in MIRRORS:
+        cls = getattr(capi, py)
+        assert got[(py, "__sizeof__")] == C.sizeof(cls), py
+        for name, _ in cls._fields_:
+            assert got[(py, name)] == getattr(cls, name).offset, (py, name)
     assert C.sizeof(capi.Batch) == 64
-    assert C.sizeof(capi.View) == 4 * 64 + 16
-    assert C.sizeof(capi.RenderParams) == 52
 
 
 def test_version_and_device_count():
