@@ -4,11 +4,16 @@
 Workload: a city-scale cloud of 1e9 points (256 box buildings sampled on
 their faces over a 2 km fBm terrain), generated on the device, Morton-sorted
 into 10,240-point batches with 30-bit coordinates in three 10-bit planes.
-One step = one frame of a 64-view seeded flythrough at 250 m: clear, cull +
-precision select, rasterise (64-bit atomicMin), resolve to RGBA8.
+One step = one frame of a 64-view seeded flythrough 100 m above the ground:
+clear, cull + precision select, rasterise (64-bit atomicMin), resolve to
+RGBA8.
 
-Prints ONE JSON line (metric, value, ms_per_step, p50/p99 frame time,
-roofline of the raster kernel, CPU-oracle baseline on the host cores).
+Prints ONE JSON line: metric, value, ms_per_step, p50/p99 frame time, the
+raster kernel's roofline, the frame counters (points drawn, skipped as
+occluded, atomics issued, pixels visible), the CPU-oracle baseline on the
+host cores, and the other legs: the 250 m flythrough, a frame with the whole
+city in view, batched multi-view, stereo hqs, hqs on the 1M-point cube,
+serving with continuous batching, ingest.
 """
 from __future__ import annotations
 
@@ -16,6 +21,7 @@ import argparse
 import json
 import math
 import os
+import platform
 import sys
 import time
 
@@ -28,12 +34,18 @@ import compute_rasterizer_amd as pcr  # noqa: E402
 
 # HBM bytes per launch of the default raster kernel on the default workload,
 # from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/
-# (a counter pass cannot run inside the timed loop)
+# over the same 64 views (a counter pass cannot run inside the timed loop)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "raster_tiled_city1e9_1080p_traffic.json")
+
+HBM_PEAK_GBS = 8000.0
+FRAME_BYTES_PER_PIXEL = 24   # clear 8, resolve: key 8 + colour gather 4 + RGBA8 write 4
+HQS_FRAME_BYTES_PER_PIXEL = 40   # depth clear 4 + sums clear 16, resolve: sums 16 + write 4
 
 
 def measured_traffic(kernel: str, config: dict):
-    """Counter-measured HBM bytes per launch for this kernel and workload, or None."""
+    """Counter-measured HBM bytes per launch and the algorithmic bytes of the
+    same profiled launches, or None when the profile is of another kernel or
+    workload."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             d = json.load(f)
@@ -42,25 +54,123 @@ def measured_traffic(kernel: str, config: dict):
     want = d.get("config", {})
     if d.get("kernel") != kernel or any(config.get(k) != v for k, v in want.items()):
         return None
-    return int(d["traffic_bytes_per_launch"])
-
-HBM_PEAK_GBS = 8000.0
+    return d
 
 
-def flythrough(n, width, height, seed=42, extent=2000.0, altitude=250.0):
-    """Seeded circular flythrough above the city, looking ahead and down."""
+def city_cfg(n_points):
+    return pcr.gen_cfg(pcr.SCENE_CITY, n_points, seed=42, num_buildings=256, terrain_frac=0.2)
+
+
+def flythrough(n, width, height, cfg, seed=42, agl=100.0, altitude=None):
+    """Seeded circular flythrough over the city, looking ahead and down: the
+    eye agl m above the ground (altitude=None) or at a fixed altitude above
+    the datum, +-10 m."""
     rng = np.random.default_rng(seed)
     views = []
-    c = extent / 2
+    c = cfg.extent / 2
     for k in range(n):
         a = 2 * math.pi * k / n + rng.uniform(-0.02, 0.02)
         r = 650.0 + rng.uniform(-40, 40)
-        eye = (c + r * math.cos(a), c + r * math.sin(a), altitude + rng.uniform(-10, 10))
+        ex, ey = c + r * math.cos(a), c + r * math.sin(a)
+        dz = rng.uniform(-10, 10)
         ahead = a + math.pi / 2 + rng.uniform(-0.3, 0.3)
-        tgt = (eye[0] + 400 * math.cos(ahead), eye[1] + 400 * math.sin(ahead), 60.0)
+        tx, ty = ex + 400 * math.cos(ahead), ey + 400 * math.sin(ahead)
+        if altitude is None:
+            eye = (ex, ey, pcr.ground_height(cfg, ex, ey) + agl + dz)
+            tgt = (tx, ty, pcr.ground_height(cfg, tx, ty) + 20.0)
+        else:
+            eye, tgt = (ex, ey, altitude + dz), (tx, ty, 60.0)
         views.append(pcr.look_at(eye, tgt, up=(0, 0, 1), fovy=60.0, near=1.0, far=5000.0, width=width,
                                  height=height))
     return views
+
+
+def overview_view(width, height, cfg):
+    """The whole city in view: looking straight down from above its centre,
+    the 2 km square (plus 5%) spanning the image height."""
+    c = cfg.extent / 2
+    d = 1.05 * c / math.tan(math.radians(30.0))
+    top = cfg.amplitude + 80.0          # highest ground plus the tallest building
+    return pcr.look_at((c, c, top + d), (c, c, 0.0), up=(0, 1, 0), fovy=60.0, near=1.0, far=5000.0, width=width,
+                       height=height)
+
+
+def frame_counters(ctx, cloud, views, prm):
+    """Per-view counters of one frame each (pcr_stats with debug counters)."""
+    out = []
+    fb = ctx.framebuffer(views[0].width, views[0].height)
+    for v in views:
+        ctx.render(cloud, v, prm, fb)
+        ctx.resolve(cloud, prm, fb)
+        st = ctx.stats()
+        drawn = st.batches_total - st.batches_culled
+        planes = st.bytes_algorithmic - 64 * st.batches_total
+        out.append({"drawn_batches": drawn, "points_drawn": st.points_drawn, "points_occluded": st.points_occluded,
+                    # raster, algorithmic: headers of the frustum-visible batches, a level byte each, and the
+                    # planes their precision levels need; read: less the planes of batches the occlusion test
+                    # skipped
+                    "raster_bytes": 64 * drawn + drawn + planes,
+                    "raster_bytes_read": 64 * drawn + drawn + planes - st.bytes_occluded,
+                    "frame_bytes": st.bytes_algorithmic + FRAME_BYTES_PER_PIXEL * v.width * v.height,
+                    "atomics": st.atomics_issued, "pixels_visible": st.pixels_visible,
+                    "levels": list(st.batches_per_level)})
+    fb.close()
+    return out
+
+
+def timed_frames(ctx, cloud, views, prm_timed, fb, frames, warmup):
+    """Wall time and per-frame raster times (HIP events around the raster
+    stage) of `frames` frames cycling through the views."""
+    for k in range(max(warmup, 1)):
+        ctx.render(cloud, views[k % len(views)], prm_timed, fb)
+        ctx.resolve(cloud, prm_timed, fb)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(frames):
+        ctx.render(cloud, views[k % len(views)], prm_timed, fb)
+        ctx.resolve(cloud, prm_timed, fb)
+    ctx.synchronize()
+    return time.perf_counter() - t0, ctx.frame_times(frames)
+
+
+def second_flythrough_leg(ctx, cloud, cfg, n_points, width, height, frames, tile):
+    """The flythrough at a fixed 250 m above the datum (the previous default
+    workload): frame time, raster time and counters."""
+    views = flythrough(64, width, height, cfg, altitude=250.0)
+    prm = pcr.params(pcr.MODE_LAS2, tile_px=tile, debug_counters=True)
+    cnt = frame_counters(ctx, cloud, views, prm)
+    fb = ctx.framebuffer(width, height)
+    wall, ft = timed_frames(ctx, cloud, views, pcr.params(pcr.MODE_LAS2, tile_px=tile, time_stages=2), fb, frames, 8)
+    fb.close()
+    idx = [k % len(views) for k in range(frames)]
+    raster = ft[:, 2].astype(np.float64)
+    return {"altitude_m": 250.0, "frames": frames, "ms_per_frame": round(wall * 1e3 / frames, 4),
+            "Gpoints_s": round(n_points * frames / wall / 1e9, 1),
+            "raster_ms_mean": round(float(raster.mean()), 4),
+            "points_drawn_mean": int(np.mean([cnt[i]["points_drawn"] for i in idx])),
+            "points_occluded_mean": int(np.mean([cnt[i]["points_occluded"] for i in idx])),
+            "raster_GBs": round(sum(cnt[i]["raster_bytes"] for i in idx) / (raster.sum() / 1e3) / 1e9, 1)}
+
+
+def overview_leg(ctx, cloud, cfg, n_points, width, height, frames, tile):
+    """One frame with the whole city in view (every batch drawn): the shape of
+    the reference's 1e9-points-per-frame figure."""
+    v = overview_view(width, height, cfg)
+    prm = pcr.params(pcr.MODE_LAS2, tile_px=tile, debug_counters=True)
+    cnt = frame_counters(ctx, cloud, [v], prm)[0]
+    fb = ctx.framebuffer(width, height)
+    wall, ft = timed_frames(ctx, cloud, [v], pcr.params(pcr.MODE_LAS2, tile_px=tile, time_stages=2), fb, frames, 4)
+    fb.close()
+    raster = ft[:, 2].astype(np.float64)
+    return {"frames": frames, "ms_per_frame": round(wall * 1e3 / frames, 4),
+            "Gpoints_s": round(n_points * frames / wall / 1e9, 1), "raster_ms_mean": round(float(raster.mean()), 4),
+            "batches_drawn": int(cnt["drawn_batches"]), "points_drawn": int(cnt["points_drawn"]),
+            "points_occluded": int(cnt["points_occluded"]), "pixels_visible": int(cnt["pixels_visible"]),
+            "atomics_issued": int(cnt["atomics"]), "levels": cnt["levels"],
+            "roofline_raster": {"achieved": round(cnt["raster_bytes"] / (raster.mean() / 1e3) / 1e9, 1),
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(cnt["raster_bytes"] / (raster.mean() / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                                "bytes_per_launch": int(cnt["raster_bytes"])}}
 
 
 MV_SIZES = [(256, 256), (640, 480), (1280, 720), (1920, 1080), (2560, 1440)]
@@ -81,7 +191,9 @@ def multiview_views(n, seed=7, extent=2000.0):
 
 
 def multiview_leg(ctx, cloud, n_points, nviews, frames, tile_px):
-    """Plain per-view frames vs one autotuned persistent launch for all views."""
+    """Plain per-view frames vs one autotuned persistent launch for all views,
+    with both forms' algorithmic bytes (the batched form reads each batch's
+    planes once, at the most precise level any view needs)."""
     views = multiview_views(nviews)
     fbs = [ctx.framebuffer(v.width, v.height) for v in views]
     prm = pcr.params(pcr.MODE_LAS2, tile_px=tile_px)
@@ -111,20 +223,131 @@ def multiview_leg(ctx, cloud, n_points, nviews, frames, tile_px):
         mv.launch()
     ctx.synchronize()
     mv_s = (time.perf_counter() - t0) / frames
+    st = mv.stats()
     exact = all(np.array_equal(fbs[k].keys(), ref[k][0]) and np.array_equal(fbs[k].rgba(), ref[k][1])
                 for k in range(nviews))
     mv.close()
     for fb in fbs:
         fb.close()
     pix = sum(v.width * v.height for v in views)
+    batched_bytes = st.bytes_batched + FRAME_BYTES_PER_PIXEL * pix
+    per_view_bytes = st.bytes_per_view_sum + FRAME_BYTES_PER_PIXEL * pix
+
+    def roof(b, s):
+        return {"bound": "hbm", "achieved": round(b / s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(b / s / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_frame": int(b)}
+
     return {"views": nviews, "megapixels": round(pix / 1e6, 2), "frames": frames,
             "plain_ms_per_frame": round(plain_s * 1e3, 3), "batched_ms_per_frame": round(mv_s * 1e3, 3),
             "plain_Gpoints_s": round(n_points * nviews / plain_s / 1e9, 1),
             "batched_Gpoints_s": round(n_points * nviews / mv_s / 1e9, 1),
+            "batched_drawn_Gpoints_s": round(st.points_per_view_sum / mv_s / 1e9, 1),
             "speedup": round(plain_s / mv_s, 3),
+            "batches_any_view": int(st.batches_any_view),
+            "roofline_batched": roof(batched_bytes, mv_s),
+            "roofline_plain_per_view": roof(per_view_bytes, plain_s),
             "tune": {"block_size": best.block_size, "points_per_thread": best.points_per_thread,
                      "tile_px": best.tile_px, "median_ms": round(best_ms, 3), "sweep_s": round(tune_s, 2)},
             "bit_exact_vs_plain": bool(exact)}
+
+
+def hqs_bytes(st, width, height):
+    """Algorithmic bytes of one hqs frame from its counters: both passes read
+    the frustum-visible batches' headers and planes, pass 2 also their
+    colours; the framebuffer adds 40 B per pixel."""
+    drawn = st.batches_total - st.batches_culled
+    planes = st.bytes_algorithmic - 64 * st.batches_total
+    return 2 * (64 * drawn + planes) + 4 * st.points_drawn + HQS_FRAME_BYTES_PER_PIXEL * width * height
+
+
+def stereo_hqs_leg(ctx, cloud, n_points, frames, width, height):
+    """hqs for two eyes (the VR methods): per-eye renders vs one shared-decode
+    multi-view launch that reads each batch once for both eyes."""
+    pairs = stereo_pairs(frames, width, height)
+    prm = pcr.params(pcr.MODE_HQS, tile_px=2048)
+    plain_fbs = [ctx.framebuffer(width, height, pcr.FB_HQS) for _ in range(2)]
+    mv_fbs = [ctx.framebuffer(width, height, pcr.FB_HQS) for _ in range(2)]
+    eye_bytes, occluded = [], []
+    for pair in pairs:                      # counters of every eye (untimed)
+        for v, fb in zip(pair, plain_fbs):
+            ctx.render(cloud, v, prm, fb)
+            st = ctx.stats()
+            eye_bytes.append(hqs_bytes(st, width, height))
+            occluded.append(st.points_occluded / max(1, 2 * st.points_drawn))
+
+    def plain_frame(pair):
+        for v, fb in zip(pair, plain_fbs):
+            ctx.render(cloud, v, prm, fb)
+            ctx.resolve(cloud, prm, fb)
+
+    plain_frame(pairs[0])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for pair in pairs:
+        plain_frame(pair)
+    ctx.synchronize()
+    plain_s = (time.perf_counter() - t0) / frames
+    best, best_ms = pcr.autotune(ctx, cloud, pairs[0], mv_fbs, prm, iters=3)
+    mv = pcr.MultiView(ctx, cloud, pairs[0], mv_fbs, prm, tune=best)
+    mv_bytes = 0
+    for pair in pairs:                      # shared-form bytes of every frame (untimed)
+        mv.set_views(pair)
+        mv.launch()
+        st = mv.stats()
+        mv_bytes += 2 * st.bytes_batched + 4 * st.points_any_view + 2 * HQS_FRAME_BYTES_PER_PIXEL * width * height
+    mv_bytes /= len(pairs)
+    t0 = time.perf_counter()
+    for pair in pairs:
+        mv.set_views(pair)
+        mv.launch()
+    ctx.synchronize()
+    mv_s = (time.perf_counter() - t0) / frames
+    exact = True
+    for p, m in zip(plain_fbs, mv_fbs):   # both hold the last pair
+        pd, pa = p.hqs()
+        md, ma = m.hqs()
+        exact = exact and bool(np.array_equal(pd, md) and np.array_equal(pa, ma) and
+                               np.array_equal(p.rgba(), m.rgba()))
+    mv.close()
+    for fb in plain_fbs + mv_fbs:
+        fb.close()
+    per_frame = float(np.mean(eye_bytes)) * 2
+    return {"eyes": 2, "width": width, "height": height, "frames": frames,
+            "per_eye_ms_per_frame": round(plain_s * 1e3, 3), "shared_decode_ms_per_frame": round(mv_s * 1e3, 3),
+            "shared_decode_Gpoints_s": round(n_points * 2 / mv_s / 1e9, 1), "speedup": round(plain_s / mv_s, 3),
+            "occluded_fraction_of_pass_points": round(float(np.mean(occluded)), 3),
+            "roofline_per_eye": {"bound": "hbm", "achieved": round(per_frame / plain_s / 1e9, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(per_frame / plain_s / 1e9 / HBM_PEAK_GBS, 4),
+                                 "bytes_per_frame": int(per_frame)},
+            "roofline_shared": {"bound": "hbm", "achieved": round(mv_bytes / mv_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": round(mv_bytes / mv_s / 1e9 / HBM_PEAK_GBS, 4),
+                                "bytes_per_frame": int(mv_bytes)},
+            "tune": {"block_size": best.block_size, "points_per_thread": best.points_per_thread,
+                     "tile_px": best.tile_px, "median_ms": round(best_ms, 3)},
+            "bit_exact_vs_per_eye": exact}
+
+
+def hqs_cube_leg(ctx, frames, width=1920, height=1080):
+    """hqs of the 1M-point cube (16 B/pt, unsorted) at 1920x1080: both passes
+    read 16 B per point (its parity with the oracle is a GPU test)."""
+    cfg = pcr.gen_cfg(pcr.SCENE_CUBE, 1_000_000, seed=1)
+    cloud = ctx.generate(cfg, morton_sort=False, keep_points16=True)
+    v = pcr.look_at((0.5, 0.5, 2.5), (0.5, 0.5, 0.5), up=(0, 1, 0), fovy=60.0, near=0.1, far=1000.0, width=width,
+                    height=height)
+    fb = ctx.framebuffer(width, height, pcr.FB_HQS)
+    prm_t = pcr.params(pcr.MODE_HQS16, time_stages=2)
+    wall, ft = timed_frames(ctx, cloud, [v], prm_t, fb, frames, 8)
+    fb.close()
+    cloud.close()
+    raster = ft[:, 2].astype(np.float64)
+    nbytes = 32 * cfg.num_points + HQS_FRAME_BYTES_PER_PIXEL * width * height
+    return {"points": cfg.num_points, "width": width, "height": height, "frames": frames,
+            "ms_per_frame": round(wall * 1e3 / frames, 4), "passes_ms_mean": round(float(raster.mean()), 4),
+            "Gpoints_s": round(cfg.num_points * frames / wall / 1e9, 2),
+            "roofline": {"bound": "hbm", "achieved": round(nbytes / (wall / frames) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(nbytes / (wall / frames) / 1e9 / HBM_PEAK_GBS, 4),
+                         "bytes_per_frame": int(nbytes), "kernels": "k_hqs_points16<0>, <1>, resolve"}}
 
 
 def stereo_pairs(n, width, height, ipd=0.065, seed=3, extent=2000.0, altitude=250.0):
@@ -143,53 +366,6 @@ def stereo_pairs(n, width, height, ipd=0.065, seed=3, extent=2000.0, altitude=25
         pairs.append([pcr.look_at(tuple(head + s * right), tuple(head + s * right + fwd), up=(0, 0, 1), fovy=90.0,
                                   near=0.1, far=5000.0, width=width, height=height) for s in (-1.0, 1.0)])
     return pairs
-
-
-def stereo_hqs_leg(ctx, cloud, n_points, frames, width, height):
-    """hqs for two eyes (the VR methods): per-eye renders vs one shared-decode
-    multi-view launch that reads each batch once for both eyes."""
-    pairs = stereo_pairs(frames, width, height)
-    prm = pcr.params(pcr.MODE_HQS, tile_px=2048)
-    plain_fbs = [ctx.framebuffer(width, height, pcr.FB_HQS) for _ in range(2)]
-    mv_fbs = [ctx.framebuffer(width, height, pcr.FB_HQS) for _ in range(2)]
-
-    def plain_frame(pair):
-        for v, fb in zip(pair, plain_fbs):
-            ctx.render(cloud, v, prm, fb)
-            ctx.resolve(cloud, prm, fb)
-
-    plain_frame(pairs[0])
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for pair in pairs:
-        plain_frame(pair)
-    ctx.synchronize()
-    plain_s = (time.perf_counter() - t0) / frames
-    best, best_ms = pcr.autotune(ctx, cloud, pairs[0], mv_fbs, prm, iters=3)
-    mv = pcr.MultiView(ctx, cloud, pairs[0], mv_fbs, prm, tune=best)
-    mv.launch()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for pair in pairs:
-        mv.set_views(pair)
-        mv.launch()
-    ctx.synchronize()
-    mv_s = (time.perf_counter() - t0) / frames
-    exact = True
-    for p, m in zip(plain_fbs, mv_fbs):   # both hold the last pair
-        pd, pa = p.hqs()
-        md, ma = m.hqs()
-        exact = exact and bool(np.array_equal(pd, md) and np.array_equal(pa, ma) and
-                               np.array_equal(p.rgba(), m.rgba()))
-    mv.close()
-    for fb in plain_fbs + mv_fbs:
-        fb.close()
-    return {"eyes": 2, "width": width, "height": height, "frames": frames,
-            "per_eye_ms_per_frame": round(plain_s * 1e3, 3), "shared_decode_ms_per_frame": round(mv_s * 1e3, 3),
-            "shared_decode_Gpoints_s": round(n_points * 2 / mv_s / 1e9, 1), "speedup": round(plain_s / mv_s, 3),
-            "tune": {"block_size": best.block_size, "points_per_thread": best.points_per_thread,
-                     "tile_px": best.tile_px, "median_ms": round(best_ms, 3)},
-            "bit_exact_vs_per_eye": exact}
 
 
 def layout_sweep(ctx, n_points, batch_sizes, nviews, frames):
@@ -221,29 +397,21 @@ def layout_sweep(ctx, n_points, batch_sizes, nviews, frames):
     return res
 
 
-def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
-    """Poisson(rate) render requests against device-resident LOD clouds, served
-    with continuous batching under a framebuffer budget; p50/p99 latency."""
+def poisson_phase(srv, ids, n_clouds, rate, seconds, seed):
+    """Poisson(rate) requests for `seconds`, each for a random cloud, view and
+    resolution; latencies from submit to image ready (the server's clock)."""
     import threading
-    clouds, build_s = [], []
-    for k, n in enumerate(sizes):
-        t = time.perf_counter()
-        cfg = pcr.gen_cfg(pcr.SCENE_CITY, n, seed=200 + k, num_buildings=256, terrain_frac=0.2)
-        clouds.append(ctx.octree_generate(cfg, seed=k + 1))
-        build_s.append(round(time.perf_counter() - t, 2))
-    srv = pcr.Server(ctx, budget_bytes=int(budget_gb * (1 << 30)), max_batch=16, num_streams=4)
-    ids = [srv.add_cloud(c) for c in clouds]
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16))
     arrivals = arrivals[arrivals < seconds]
     n_req = len(arrivals)
-    which = rng.integers(0, len(clouds), n_req)
+    which = rng.integers(0, n_clouds, n_req)
     views = multiview_views(n_req, seed=seed + 1)
     tickets = [None] * n_req
     ready = threading.Semaphore(0)
-    start = time.perf_counter()
-
     errors = []
+    s0 = srv.stats()
+    start = time.perf_counter()
 
     def producer():
         try:
@@ -265,42 +433,75 @@ def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
             raise RuntimeError(f"serving: request {i} was not submitted within 60 s")
         if errors:
             raise errors[0]
-        _, ms = srv.wait(tickets[i], views[i].width * views[i].height)
+        _, ms = srv.wait(tickets[i])
         lat.append(ms)
     wall = time.perf_counter() - start
     th.join()
-    st = srv.stats()
+    s1 = srv.stats()
+    lat = np.array(lat)
+    launches = s1.launches - s0.launches
+    return {"rate_per_s": rate, "requests": n_req, "seconds": round(wall, 2),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+            "latency_ms_max": round(float(lat.max()), 3), "launches": int(launches),
+            "mean_requests_per_launch": round(n_req / max(1, launches), 2),
+            "Gpoints_s": round((s1.points_rendered - s0.points_rendered) / wall / 1e9, 2),
+            "drawn_Gpoints_s": round((s1.points_drawn - s0.points_drawn) / wall / 1e9, 3),
+            "algorithmic_GBs": round((s1.bytes_algorithmic - s0.bytes_algorithmic) / wall / 1e9, 2)}
 
-    # Saturated phase: the same kind of requests all submitted at once, served
-    # with continuous batching, against rendering them one at a time through
-    # the plain per-view path. Both hand every image to the host; a sample of
-    # images is compared bit for bit.
-    nb = min(n_req, 512)
+
+def serving_leg(ctx, seconds, rates, budget_gb, sizes, seed=11):
+    """Render requests against device-resident LOD clouds, served with
+    continuous batching under a framebuffer budget: Poisson arrivals at each
+    rate (p50/p99 latency), then a burst against the plain per-view path."""
+    clouds, build_s = [], []
+    for k, n in enumerate(sizes):
+        t = time.perf_counter()
+        cfg = pcr.gen_cfg(pcr.SCENE_CITY, n, seed=200 + k, num_buildings=256, terrain_frac=0.2)
+        clouds.append(ctx.octree_generate(cfg, seed=k + 1))
+        build_s.append(round(time.perf_counter() - t, 2))
+    srv = pcr.Server(ctx, budget_bytes=int(budget_gb * (1 << 30)), max_batch=16, num_streams=4)
+    ids = [srv.add_cloud(c) for c in clouds]
+    phases = [poisson_phase(srv, ids, len(clouds), rate, seconds if k == 0 else min(seconds, 2.0), seed + 10 * k)
+              for k, rate in enumerate(rates)]
+
+    # Burst: requests all submitted at once, served with continuous batching,
+    # against rendering them one at a time through the plain per-view path.
+    # Both hand every image to the host; a sample of images is compared bit
+    # for bit.
+    rng = np.random.default_rng(seed)
+    nb = 512
     bviews = multiview_views(nb, seed=seed + 2)
     bwhich = rng.integers(0, len(clouds), nb)
     prm = pcr.params(pcr.MODE_NODES, lod_px=1.0)
     fbs = {}
+    drawn = 0
+    for i in range(nb):                     # counters of the plain renders (untimed)
+        v = bviews[i]
+        fb = fbs.get((v.width, v.height)) or fbs.setdefault((v.width, v.height), ctx.framebuffer(v.width, v.height))
+        ctx.render(clouds[bwhich[i]], v, prm, fb)
+        drawn += ctx.stats().points_drawn
     t0 = time.perf_counter()
     for i in range(nb):
         v = bviews[i]
-        fb = fbs.get((v.width, v.height)) or fbs.setdefault((v.width, v.height), ctx.framebuffer(v.width, v.height))
+        fb = fbs[(v.width, v.height)]
         ctx.render(clouds[bwhich[i]], v, prm, fb)
         ctx.resolve(clouds[bwhich[i]], prm, fb)
         fb.rgba()
     ctx.synchronize()
     plain_s = time.perf_counter() - t0
-    launches0 = srv.stats().launches
+    s0 = srv.stats()
     t0 = time.perf_counter()
     bt = [srv.submit(ids[bwhich[i]], bviews[i], pcr.MODE_NODES, lod_px=1.0) for i in range(nb)]
     imgs = {}
     for i, t in enumerate(bt):
         # zero copy: the image is read in the server's pinned result memory
-        img, _ = srv.wait_pinned(t, bviews[i].width * bviews[i].height)
+        img, _ = srv.wait_pinned(t)
         if i % 64 == 0:
             imgs[i] = img.copy()
         srv.release(t)
     served_s = time.perf_counter() - t0
-    burst_launches = srv.stats().launches - launches0
+    s1 = srv.stats()
     exact = True
     for i, img in imgs.items():
         v = bviews[i]
@@ -311,23 +512,25 @@ def serving_leg(ctx, seconds, rate, budget_gb, sizes, seed=11):
     for fb in fbs.values():
         fb.close()
     pts = float(sum(int(clouds[k].info().num_points) for k in bwhich))
+    pix = float(sum(v.width * v.height for v in bviews))
+    served_bytes = (s1.bytes_algorithmic - s0.bytes_algorithmic) + FRAME_BYTES_PER_PIXEL * pix
     burst = {"requests": nb, "plain_s": round(plain_s, 3), "served_s": round(served_s, 3),
              "plain_Gpoints_s": round(pts / plain_s / 1e9, 1), "served_Gpoints_s": round(pts / served_s / 1e9, 1),
-             "speedup": round(plain_s / served_s, 3), "launches": int(burst_launches),
-             "mean_requests_per_launch": round(nb / max(1, burst_launches), 2),
+             "drawn_points": int(drawn), "served_drawn_Gpoints_s": round(drawn / served_s / 1e9, 2),
+             "speedup": round(plain_s / served_s, 3), "launches": int(s1.launches - s0.launches),
+             "mean_requests_per_launch": round(nb / max(1, s1.launches - s0.launches), 2),
+             "roofline": {"bound": "hbm", "achieved": round(served_bytes / served_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(served_bytes / served_s / 1e9 / HBM_PEAK_GBS, 4),
+                          "bytes": int(served_bytes),
+                          "note": "whole served job: node headers and planes per launch plus 24 B/px, over wall time"},
              "bit_exact_vs_plain_sample": exact}
+    st = srv.stats()
     srv.close()
     for c in clouds:
         c.close()
-    lat = np.array(lat)
-    return {"burst": burst, "clouds": list(sizes), "octree_build_s": build_s, "requests": n_req, "rate_per_s": rate,
-            "seconds": round(wall, 2), "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
-            "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
-            "latency_ms_max": round(float(lat.max()), 3),
-            "Gpoints_s": round(st.points_rendered / wall / 1e9, 2), "launches": int(st.launches),
-            "mean_requests_per_launch": round(n_req / max(1, st.launches), 2),
-            "fb_budget_GB": budget_gb, "fb_peak_GB": round(st.fb_bytes_peak / (1 << 30), 3),
-            "lod_px": 1.0, "mode": "nodes"}
+    return {"clouds": list(sizes), "octree_build_s": build_s, "fb_budget_GB": budget_gb,
+            "fb_peak_GB": round(st.fb_bytes_peak / (1 << 30), 3), "lod_px": 1.0, "mode": "nodes",
+            "poisson": phases, "burst": burst}
 
 
 def ingest_leg(ctx, n_las, seed=5):
@@ -373,26 +576,51 @@ def ingest_leg(ctx, n_las, seed=5):
     return out
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(ctx, cloud, views, gpu_fb, prm, frames, n_points):
-    """Oracle (multithreaded C) on the host cores for `frames` full frames;
-    the GPU framebuffers of the same views are compared bit for bit."""
+    """The oracle's renderer (multithreaded C) on the host cores: a warm-up
+    frame, then the median of `frames` frames with all threads of this GPU's
+    share of the host, each compared bit for bit with the GPU's framebuffer,
+    and the median of `frames` frames on one thread."""
     from oracle import orc
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
+    avail = len(os.sched_getaffinity(0))
+    cores = min(avail, 16)   # a one-GPU box gives a job 16 host cores, whatever the node has
     lay = cloud.download()
+    b, lo, med, hi = lay["batches"], lay["lo"], lay["med"], lay["hi"]
+
+    def run(v, threads):
+        t0 = time.perf_counter()
+        ref = orc.render_batched_mt(b, lo, med, hi, v, 1, threads)
+        return time.perf_counter() - t0, ref
+
+    run(views[0], cores)   # warm-up: pages the 16 GB layout in
     times, exact = [], True
     for j in range(frames):
-        v = views[j]
-        t0 = time.perf_counter()
-        ref = orc.render_batched_mt(lay["batches"], lay["lo"], lay["med"], lay["hi"], v, 1, cores)
-        times.append(time.perf_counter() - t0)
+        v = views[1 + j]
+        dt, ref = run(v, cores)
+        times.append(dt)
         ctx.render(cloud, v, prm, gpu_fb)
         exact = exact and bool(np.array_equal(gpu_fb.keys(), ref))
-    del lay
-    cpu_s = float(np.median(times))
+    t1 = [run(views[1 + j], 1)[0] for j in range(frames)]
+    del lay, b, lo, med, hi
+    cpu_s, one_s = float(np.median(times)), float(np.median(t1))
     return {"value": round(n_points / cpu_s / 1e9, 4), "unit": "Gpoints/s", "cores": cores, "kind": "port",
-            "sample": f"{frames} full 1920x1080 frames of the 1e9-point cloud (median {cpu_s:.2f} s/frame), "
-                      f"oracle/pcr_oracle.c multithreaded fetch-min, framebuffers bit-exact to the GPU: {exact}"}
+            "cores_available": avail, "cpu_model": cpu_model(),
+            "value_1thread": round(n_points / one_s / 1e9, 4),
+            "ms_per_frame": round(cpu_s * 1e3, 1), "ms_per_frame_1thread": round(one_s * 1e3, 1),
+            "sample": f"1 warm-up then {frames} full 1920x1080 frames of the 1e9-point cloud on {cores} threads "
+                      f"(median {cpu_s:.3f} s/frame) and {frames} on 1 thread (median {one_s:.2f} s/frame), "
+                      f"oracle/pcr_oracle.c fetch-min renderer; framebuffers bit-exact to the GPU: {exact}"}
 
 
 def main():
@@ -404,17 +632,20 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--views", type=int, default=64)
+    ap.add_argument("--agl", type=float, default=100.0, help="flythrough height above the ground (m)")
     ap.add_argument("--mode", default="las2", choices=["las", "las2"])
-    ap.add_argument("--cpu-frames", type=int, default=2, help="0 skips the CPU baseline")
+    ap.add_argument("--cpu-frames", type=int, default=5, help="0 skips the CPU baseline")
     ap.add_argument("--tile-px", type=int, default=2048, help="LDS tile size of the tiled raster (0 = direct)")
     ap.add_argument("--block", type=int, default=0, help="raster workgroup size (0 = default)")
+    ap.add_argument("--extra-frames", type=int, default=32, help="frames of the 250 m and overview legs (0 skips)")
     ap.add_argument("--mv-views", type=int, default=48, help="views of the batched multi-view leg (0 skips it)")
     ap.add_argument("--mv-frames", type=int, default=8)
     ap.add_argument("--stereo-frames", type=int, default=16, help="frames of the two-eye hqs leg (0 skips it)")
+    ap.add_argument("--cube-hqs-frames", type=int, default=64, help="frames of the 1M-point cube hqs leg (0 skips)")
     ap.add_argument("--mv-batch-sizes", default="5120,20480",
                     help="extra layouts for the multi-view autotune sweep ('' skips it)")
-    ap.add_argument("--serve-seconds", type=float, default=8.0, help="serving leg duration (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="first serving phase duration (0 skips it)")
+    ap.add_argument("--serve-rates", default="200,1000,2000,3000", help="Poisson arrival rates of the serving phases")
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
     ap.add_argument("--serve-clouds", default="10e6,100e6,500e6,1e9")
     ap.add_argument("--ingest-points", type=float, default=20e6, help="LAS ingest leg size (0 skips the leg)")
@@ -441,7 +672,7 @@ def main():
 
     ctx = pcr.Context(0)
     t = time.perf_counter()
-    cfg = pcr.gen_cfg(pcr.SCENE_CITY, args.points, seed=42, num_buildings=256, terrain_frac=0.2)
+    cfg = city_cfg(args.points)
     cloud = ctx.generate(cfg)
     gen_s = time.perf_counter() - t
     info = cloud.info()
@@ -449,7 +680,7 @@ def main():
           file=sys.stderr, flush=True)
 
     W, H = args.width, args.height
-    views = flythrough(args.views, W, H)
+    views = flythrough(args.views, W, H, cfg, agl=args.agl)
     mode = pcr.MODE_LAS2 if args.mode == "las2" else pcr.MODE_LAS
     tile = args.tile_px if mode == pcr.MODE_LAS2 else 0
     prm = pcr.params(mode, time_stages=0, tile_px=tile, block_size=args.block)
@@ -458,40 +689,15 @@ def main():
     # resolve stage times come from a separate pass with all six events
     prm_timed = pcr.params(mode, time_stages=2, tile_px=tile, block_size=args.block)
     prm_stages = pcr.params(mode, time_stages=1, tile_px=tile, block_size=args.block)
+    cnt = frame_counters(ctx, cloud, views, pcr.params(mode, tile_px=tile, block_size=args.block,
+                                                       debug_counters=True))
+    hist = np.sum([c["levels"] for c in cnt], axis=0)
+    print(f"[bench] per-view counters done; level histogram {hist.tolist()}", file=sys.stderr, flush=True)
     fb = ctx.framebuffer(W, H)
-
-    # per-view algorithmic bytes of the raster kernel: headers of drawn batches,
-    # the planes their precision level needs, one level byte per batch
-    raster_bytes, frame_bytes, drawn_pts, hist = [], [], [], np.zeros(5, np.int64)
-    for v in views:
-        ctx.render(cloud, v, prm, fb)
-        ctx.resolve(cloud, prm, fb)
-        st = ctx.stats()
-        drawn = st.batches_total - st.batches_culled
-        plane = st.bytes_algorithmic - 64 * st.batches_total
-        raster_bytes.append(plane + 64 * drawn + st.batches_total)
-        # whole frame: select reads all headers; clear 8, resolve 8 + 4 gather + 4 write per pixel
-        frame_bytes.append(st.bytes_algorithmic + 24 * W * H)
-        drawn_pts.append(st.points_drawn)
-        hist += np.array(list(st.batches_per_level))
-    print(f"[bench] per-view stats done; level histogram {hist.tolist()}", file=sys.stderr, flush=True)
-
-    for k in range(max(args.warmup, 1)):
-        ctx.render(cloud, views[k % len(views)], prm_timed, fb)
-        ctx.resolve(cloud, prm_timed, fb)
-    ctx.synchronize()
-
     K = args.steps
-    t0 = time.perf_counter()
-    for k in range(K):
-        ctx.render(cloud, views[k % len(views)], prm_timed, fb)
-        ctx.resolve(cloud, prm_timed, fb)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    ft = ctx.frame_times(K)
+    wall, ft = timed_frames(ctx, cloud, views, prm_timed, fb, K, args.warmup)
     print(f"[bench] timed {K} frames in {wall:.3f} s", file=sys.stderr, flush=True)
-    # stage pass (not timed): 32 frames with events around every stage
-    for k in range(32):
+    for k in range(32):   # stage pass (not timed): events around every stage
         ctx.render(cloud, views[k % len(views)], prm_stages, fb)
         ctx.resolve(cloud, prm_stages, fb)
     st_ft = ctx.frame_times(32)
@@ -510,10 +716,26 @@ def main():
     raster_ms = ft[:, 2].astype(np.float64)
     frame_ms = ft[:, 4].astype(np.float64)   # raster end to raster end
     frame_ms = frame_ms[frame_ms > 0]
-    achieved = sum(raster_bytes[i] for i in idx) / (raster_ms.sum() / 1e3) / 1e9
-    frame_gbs = float(np.mean([frame_bytes[i] for i in idx])) / (frame_ms.mean() / 1e3) / 1e9
+    mean = lambda key: float(np.mean([cnt[i][key] for i in idx]))  # noqa: E731
+    achieved = sum(cnt[i]["raster_bytes"] for i in idx) / (raster_ms.sum() / 1e3) / 1e9
+    frame_gbs = mean("frame_bytes") / (frame_ms.mean() / 1e3) / 1e9
     value = info.num_points * K / wall / 1e9
-
+    kernel = ("k_raster_tiled<%d,%d>" % (args.block if args.block in (128, 256) else 256, args.tile_px)
+              if args.tile_px and mode == pcr.MODE_LAS2 else
+              "k_raster_batched<%d,%d>" % (args.block or (256 if mode == pcr.MODE_LAS2 else 128),
+                                          4 if mode == pcr.MODE_LAS2 else 1))
+    config = {"workload": f"city-1e9-adaptive-1080p-flythrough-{args.agl:g}m-agl", "points": info.num_points,
+              "batches": info.num_batches, "batch_size": 10240, "width": W, "height": H, "views": len(views),
+              "mode": args.mode, "tile_px": args.tile_px}
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "bytes_per_launch_mean": int(mean("raster_bytes")),
+            "bytes_read_per_launch_mean": int(mean("raster_bytes_read"))}
+    prof = measured_traffic(kernel, config)
+    if prof:
+        roof["traffic"] = int(prof["traffic_bytes_per_launch"])
+        roof["traffic_over_algorithmic"] = round(prof["traffic_bytes_per_launch"] /
+                                                 prof["algorithmic_bytes_per_launch"], 3)
     out = {
         "metric": "Gpoints/s (1e9-pt adaptive-precision 1920x1080 frames)",
         "value": round(value, 3),
@@ -527,29 +749,30 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "city-1e9-adaptive-1080p-flythrough", "points": info.num_points,
-                   "batches": info.num_batches, "batch_size": 10240, "width": W, "height": H,
-                   "views": len(views), "mode": args.mode, "tile_px": args.tile_px},
+        "config": config,
         "frame_ms_p50": round(float(np.percentile(frame_ms, 50)), 4),
         "frame_ms_p99": round(float(np.percentile(frame_ms, 99)), 4),
         "stage_ms_mean": {"clear": round(float(st_ft[:, 0].mean()), 4), "cull": round(float(st_ft[:, 1].mean()), 4),
                           "raster": round(float(raster_ms.mean()), 4),
                           "resolve": round(float(st_ft[:, 3].mean()), 4)},
         "stage_ms_source": "raster: the timed frames; clear, cull, resolve: a 32-frame pass with all stage events",
-        "points_in_drawn_batches_mean": int(np.mean([drawn_pts[i] for i in idx])),
+        "points_in_drawn_batches_mean": int(mean("points_drawn")),
+        "drawn_Gpoints_s": round(mean("points_drawn") * K / wall / 1e9, 1),
+        "points_occluded_mean": int(mean("points_occluded")),
+        "atomics_issued_mean": int(mean("atomics")),
+        "pixels_visible_mean": int(mean("pixels_visible")),
         "level_histogram_per_frame": (hist / len(views)).round(1).tolist(),
         "frame_algorithmic_GBs": round(frame_gbs, 1),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": ("k_raster_tiled<%d,%d>" % (args.block if args.block in (128, 256) else 256, args.tile_px)
-                                if args.tile_px and mode == pcr.MODE_LAS2 else
-                                "k_raster_batched<%d,%d>" % (args.block or (256 if mode == pcr.MODE_LAS2 else 128),
-                                                             4 if mode == pcr.MODE_LAS2 else 1)),
-                     "bytes_per_launch_mean": int(np.mean([raster_bytes[i] for i in idx]))},
+        "roofline": roof,
         "resolve_edl_ms": round(edl_ms, 4),
         "generate_s": round(gen_s, 2),
     }
-    out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"], out["config"])
+    if args.extra_frames > 0:
+        out["flythrough_250m"] = second_flythrough_leg(ctx, cloud, cfg, info.num_points, W, H, args.extra_frames,
+                                                       tile)
+        print(f"[bench] 250 m flythrough {out['flythrough_250m']}", file=sys.stderr, flush=True)
+        out["overview"] = overview_leg(ctx, cloud, cfg, info.num_points, W, H, args.extra_frames, tile)
+        print(f"[bench] overview {out['overview']}", file=sys.stderr, flush=True)
     if args.mv_views > 0:
         out["multiview"] = multiview_leg(ctx, cloud, info.num_points, args.mv_views, args.mv_frames, args.tile_px)
         print(f"[bench] multiview {out['multiview']}", file=sys.stderr, flush=True)
@@ -558,17 +781,20 @@ def main():
         print(f"[bench] stereo hqs {out['stereo_hqs']}", file=sys.stderr, flush=True)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(ctx, cloud, views, fb, prm, args.cpu_frames, info.num_points)
+        print(f"[bench] cpu baseline {out['cpu_baseline']}", file=sys.stderr, flush=True)
+    fb.close()
+    cloud.close()
+    if args.cube_hqs_frames > 0:
+        out["hqs_cube"] = hqs_cube_leg(ctx, args.cube_hqs_frames)
+        print(f"[bench] hqs cube {out['hqs_cube']}", file=sys.stderr, flush=True)
     if args.mv_views > 0 and args.mv_batch_sizes:
-        fb.close()
-        cloud.close()
         sizes = [int(x) for x in args.mv_batch_sizes.split(",")]
         out["multiview"]["layout_sweep"] = layout_sweep(ctx, args.points, sizes, args.mv_views, args.mv_frames)
         print(f"[bench] layout sweep {out['multiview']['layout_sweep']}", file=sys.stderr, flush=True)
     if args.serve_seconds > 0:
-        fb.close()
-        cloud.close()
         sizes = [int(float(x)) for x in args.serve_clouds.split(",")]
-        out["serving"] = serving_leg(ctx, args.serve_seconds, args.serve_rate, args.serve_budget_gb, sizes)
+        rates = [float(x) for x in args.serve_rates.split(",")]
+        out["serving"] = serving_leg(ctx, args.serve_seconds, rates, args.serve_budget_gb, sizes)
         print(f"[bench] serving {out['serving']}", file=sys.stderr, flush=True)
     if args.ingest_points > 0:
         out["ingest"] = ingest_leg(ctx, int(args.ingest_points))

@@ -75,6 +75,13 @@ def gen_cfg(scene, n, seed=1, extent=2000.0, amplitude=150.0, jitter=0.5, num_bu
     return GenCfg(scene, num_buildings, seed, n, extent, amplitude, jitter, terrain_frac)
 
 
+def ground_height(cfg: GenCfg, x: float, y: float) -> float:
+    """Ground height of a TERRAIN or CITY scene at (x, y) (pcr_scene_ground_height)."""
+    h = C.c_float()
+    check(capi.load().pcr_scene_ground_height(C.byref(cfg), x, y, C.byref(h)))
+    return h.value
+
+
 def encode_batches_host(xyz: np.ndarray, batch_size: int = POINTS_PER_BATCH):
     """Host encoder (LasLoaderSparse.cpp:37-212): returns (batches, lo, med, hi)."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
@@ -202,6 +209,12 @@ class MultiView:
 
     def launch(self):
         check(capi.load().pcr_multiview_launch(self.h))
+
+    def stats(self) -> capi.MultiViewStats:
+        """Batches and algorithmic bytes of the last launch (synchronises)."""
+        st = capi.MultiViewStats()
+        check(capi.load().pcr_multiview_stats_get(self.h, C.byref(st)))
+        return st
 
     def close(self):
         if self.h:

@@ -87,7 +87,8 @@ class Stats(C.Structure):
                 ("batches_per_level", C.c_uint64 * 5), ("points_drawn", C.c_uint64),
                 ("bytes_algorithmic", C.c_uint64), ("ms_cull", C.c_float), ("ms_raster", C.c_float),
                 ("ms_resolve", C.c_float), ("ms_clear", C.c_float), ("batches_occluded", C.c_uint64),
-                ("points_occluded", C.c_uint64), ("atomics_issued", C.c_uint64), ("pixels_visible", C.c_uint64)]
+                ("points_occluded", C.c_uint64), ("bytes_occluded", C.c_uint64), ("atomics_issued", C.c_uint64),
+                ("pixels_visible", C.c_uint64)]
 
 
 class GenCfg(C.Structure):
@@ -121,9 +122,15 @@ class Request(C.Structure):
                 ("view", View)]
 
 
+class MultiViewStats(C.Structure):
+    _fields_ = [("batches_any_view", C.c_uint64), ("bytes_batched", C.c_uint64), ("bytes_per_view_sum", C.c_uint64),
+                ("points_per_view_sum", C.c_uint64), ("points_any_view", C.c_uint64)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("completed", C.c_uint64), ("refused", C.c_uint64), ("launches", C.c_uint64),
-                ("points_rendered", C.c_uint64), ("fb_bytes_in_use", C.c_uint64), ("fb_bytes_peak", C.c_uint64)]
+                ("points_rendered", C.c_uint64), ("fb_bytes_in_use", C.c_uint64), ("fb_bytes_peak", C.c_uint64),
+                ("points_drawn", C.c_uint64), ("bytes_algorithmic", C.c_uint64)]
 
 
 P = C.c_void_p
@@ -146,6 +153,7 @@ SIGNATURES = {
     "pcr_cloud_upload_points16": (C.c_int, [P, P, P, u64, PP]),
     "pcr_cloud_upload_batched": (C.c_int, [P, P, u32, P, P, P, P, u64, PP]),
     "pcr_generate_points": (C.c_int, [P, C.POINTER(GenCfg), P, P]),
+    "pcr_scene_ground_height": (C.c_int, [C.POINTER(GenCfg), f32, f32, C.POINTER(C.c_float)]),
     "pcr_cloud_build_device": (C.c_int, [P, P, P, u64, C.POINTER(BuildOpts), PP]),
     "pcr_cloud_generate": (C.c_int, [P, C.POINTER(GenCfg), C.POINTER(BuildOpts), PP]),
     "pcr_cloud_info_get": (C.c_int, [P, C.POINTER(CloudInfo)]),
@@ -179,6 +187,7 @@ SIGNATURES = {
     "pcr_multiview_set_views": (C.c_int, [P, C.POINTER(View), C.c_int]),
     "pcr_multiview_launch": (C.c_int, [P]),
     "pcr_multiview_destroy": (C.c_int, [P]),
+    "pcr_multiview_stats_get": (C.c_int, [P, C.POINTER(MultiViewStats)]),
     "pcr_autotune": (C.c_int, [P, P, C.POINTER(View), C.c_int, C.POINTER(C.c_void_p), C.POINTER(RenderParams),
                                C.c_int, C.POINTER(Tune), C.POINTER(C.c_float)]),
     "pcr_server_create": (C.c_int, [P, u64, C.c_int, C.c_int, PP]),

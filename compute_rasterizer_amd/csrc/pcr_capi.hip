@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <vector>
+#include "pcr_generate.h"
 #include "pcr_internal.h"
 
 namespace pcr {
@@ -230,6 +231,14 @@ int pcr_cloud_upload_points16(pcr_ctx* ctx, const float* xyz, const uint32_t* rg
     for (int a = 0; a < 3; a++) { c->aabb_min[a] = mn[a]; c->aabb_max[a] = mx[a]; }
     PCR_HIP_OR(hipMemcpy(c->d_points16, packed.data(), n * 16, hipMemcpyHostToDevice), (cloud_free(c), delete c));
     *out = c;
+    return PCR_OK;
+}
+
+int pcr_scene_ground_height(const pcr_gen_cfg* cfg, float x, float y, float* height) {
+    PCR_CHECK(cfg && height, "scene_ground_height: null argument");
+    PCR_CHECK(cfg->scene == PCR_SCENE_TERRAIN || cfg->scene == PCR_SCENE_CITY,
+              "scene_ground_height: scene %d has no ground", cfg->scene);
+    *height = terrain_height(cfg->seed, cfg->amplitude, x, y);
     return PCR_OK;
 }
 
@@ -545,6 +554,7 @@ int pcr_stats_get(pcr_ctx* ctx, pcr_stats* out) {
     out->bytes_algorithmic = st.plane_bytes + 64ull * ctx->last_nb;
     out->batches_occluded = st.occluded_batches;
     out->points_occluded = st.occluded_points;
+    out->bytes_occluded = st.occluded_bytes;
     out->atomics_issued = st.atomics;
     out->pixels_visible = st.pixels_visible;
     if (ctx->timing && ctx->frames_timed) {

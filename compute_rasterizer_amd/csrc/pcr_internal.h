@@ -20,6 +20,7 @@ struct FrameState {
     unsigned long long plane_bytes;
     uint32_t occluded_batches;          // batch draws skipped by the occlusion test (rect_open)
     unsigned long long occluded_points; // their points
+    unsigned long long occluded_bytes;  // the plane (hqs pass 2: and colour) bytes they did not read
     unsigned long long atomics;         // 64-bit framebuffer atomics issued (LAS, LAS2, NODES)
     unsigned long long pixels_visible;  // pixels the index resolve found covered
 };
@@ -138,6 +139,11 @@ struct MVState {
     uint32_t list_count;   // batches drawn by at least one view
     uint32_t queue_head;   // work items taken by the persistent raster (hqs: pass 1)
     uint32_t queue_head2;  // hqs: work items taken by pass 2
+    uint32_t pad;
+    unsigned long long bytes_batched;    // headers + planes at each kept batch's most precise level
+    unsigned long long bytes_per_view;   // headers + planes summed over the views that draw each batch
+    unsigned long long points_per_view;  // points in drawn batches summed over views
+    unsigned long long points_any_view;  // points in batches drawn by at least one view
 };
 struct MVBuffers {
     pcr_view* d_views = nullptr;

@@ -55,7 +55,8 @@ __device__ __forceinline__ void frame_reset(FrameState* st) {
     st->list_count = 0; st->queue_head = 0; st->queue_head2 = 0; st->culled = 0;
     for (int i = 0; i < 5; i++) st->level_count[i] = 0;
     st->points_drawn = 0; st->plane_bytes = 0;
-    st->occluded_batches = 0; st->occluded_points = 0; st->atomics = 0; st->pixels_visible = 0;
+    st->occluded_batches = 0; st->occluded_points = 0; st->occluded_bytes = 0; st->atomics = 0;
+    st->pixels_visible = 0;
 }
 // 16-byte stores (p is hipMalloc-aligned); the last element of an odd-length
 // 64-bit fill, or the last 1-3 of a 32-bit fill, are stored singly.
@@ -569,10 +570,10 @@ __global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restr
                                                         unsigned long long* __restrict__ fb, int occl) {
     __shared__ unsigned long long tile[TILE_PIX];
     __shared__ uint32_t s_item, s_occ_batches, s_natom;   // s_natom: framebuffer atomics issued
-    __shared__ unsigned long long s_occ_points;   // batches and points the occlusion test skipped
+    __shared__ unsigned long long s_occ_points, s_occ_bytes;   // what the occlusion test skipped
     const uint32_t count = st->list_count;
     const uint32_t nsplit = min(count, gridDim.x), whole = count - nsplit;
-    if (threadIdx.x == 0) { s_occ_batches = 0; s_occ_points = 0; s_natom = 0; }
+    if (threadIdx.x == 0) { s_occ_batches = 0; s_occ_points = 0; s_occ_bytes = 0; s_natom = 0; }
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(&st->queue_head, 1u);
         __syncthreads();
@@ -597,6 +598,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restr
                                                                     occl != 0, part, nparts);
         if (threadIdx.x == 0 && skipped) {
             s_occ_points += skipped;
+            s_occ_bytes += 4ull * planes_for_level(level) * skipped;
             s_occ_batches += part == 0;   // a batch drawn as two halves counts once
         }
         __syncthreads();   // s_item and the tile are reused by the next item
@@ -606,6 +608,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restr
     if (threadIdx.x == 0 && s_occ_points) {
         atomicAdd(&st->occluded_batches, s_occ_batches);
         atomicAdd(&st->occluded_points, s_occ_points);
+        atomicAdd(&st->occluded_bytes, s_occ_bytes);
     }
 }
 
@@ -984,8 +987,8 @@ void k_hqs_tiled(const pcr_batch* __restrict__ b,
     const uint32_t count = st->list_count;
     const uint32_t nsplit = min(count, gridDim.x), whole = count - nsplit;
     __shared__ uint32_t s_occ_batches;
-    __shared__ unsigned long long s_occ_points;   // batches and points the occlusion test skipped
-    if (threadIdx.x == 0) { s_occ_batches = 0; s_occ_points = 0; }
+    __shared__ unsigned long long s_occ_points, s_occ_bytes;   // what the occlusion test skipped
+    if (threadIdx.x == 0) { s_occ_batches = 0; s_occ_points = 0; s_occ_bytes = 0; }
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
         __syncthreads();
@@ -1002,6 +1005,7 @@ void k_hqs_tiled(const pcr_batch* __restrict__ b,
                                                                               occl != 0, part, nparts);
         if (threadIdx.x == 0 && skipped) {
             s_occ_points += skipped;
+            s_occ_bytes += (4ull * planes_for_level(level) + (PASS == 1 ? 4ull : 0ull)) * skipped;
             s_occ_batches += part == 0;
         }
         __syncthreads();   // s_item and the tile are reused by the next item
@@ -1009,6 +1013,7 @@ void k_hqs_tiled(const pcr_batch* __restrict__ b,
     if (threadIdx.x == 0 && s_occ_points) {
         atomicAdd(&st->occluded_batches, s_occ_batches);
         atomicAdd(&st->occluded_points, s_occ_points);
+        atomicAdd(&st->occluded_bytes, s_occ_bytes);
     }
 }
 

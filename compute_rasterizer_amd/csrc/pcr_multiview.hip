@@ -31,7 +31,19 @@ namespace pcr {
 constexpr int kMaxViews = 64;
 
 __global__ void k_mv_reset(MVState* st) {
-    if (threadIdx.x == 0) { st->list_count = 0; st->queue_head = 0; st->queue_head2 = 0; }
+    if (threadIdx.x == 0) {
+        st->list_count = 0; st->queue_head = 0; st->queue_head2 = 0;
+        st->bytes_batched = 0; st->bytes_per_view = 0; st->points_per_view = 0; st->points_any_view = 0;
+    }
+}
+
+__device__ __forceinline__ unsigned long long mv_wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, off), hi = __shfl_xor((uint32_t)(v >> 32), off);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
 }
 
 __global__ void k_mv_clear(unsigned long long* const* __restrict__ fbs, const pcr_view* __restrict__ views) {
@@ -51,11 +63,14 @@ __global__ __launch_bounds__(256) void k_mv_select(const pcr_batch* __restrict__
                                                    uint8_t* __restrict__ levels, uint8_t* __restrict__ best,
                                                    uint32_t* __restrict__ list, MVState* __restrict__ st) {
     __shared__ float planes[kMaxViews][6][4];   // every view's frustum planes, once per workgroup
+    __shared__ unsigned long long s_bytes[4];   // batched bytes, per-view bytes, per-view points, kept points
     for (int t = (int)threadIdx.x; t < 6 * nviews; t += (int)blockDim.x)
         frustum_plane(views[t / 6].transform_frustum, t % 6, planes[t / 6][t % 6]);
+    if (threadIdx.x < 4) s_bytes[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     int top = kCulled;
+    unsigned long long view_bytes = 0, view_points = 0;
     if (i < nb) {
         const pcr_batch h = b[i];
         const float first = lod ? 80.0f : 100.0f;
@@ -79,10 +94,33 @@ __global__ __launch_bounds__(256) void k_mv_select(const pcr_batch* __restrict__
             const int level = vis ? precision_level(h.min, h.max, vw, first) : kCulled;
             levels[(uint64_t)v * nb + i] = (uint8_t)level;
             top = level < top ? level : top;
+            if (level != kCulled) {
+                view_bytes += 64ull + 4ull * planes_for_level(level) * h.num_points;
+                view_points += h.num_points;
+            }
         }
         best[i] = (uint8_t)top;
     }
     const bool keep = top != kCulled;
+    {   // algorithmic bytes of the frame: one atomic per workgroup and counter
+        const unsigned long long np = keep ? b[i].num_points : 0ull;
+        const unsigned long long bb = mv_wave_sum(keep ? 64ull + 4ull * planes_for_level(top) * np : 0ull);
+        const unsigned long long vb = mv_wave_sum(view_bytes), vp = mv_wave_sum(view_points);
+        const unsigned long long kp = mv_wave_sum(np);
+        if (__lane_id() == 0) {
+            if (bb) atomicAdd(&s_bytes[0], bb);
+            if (vb) atomicAdd(&s_bytes[1], vb);
+            if (vp) atomicAdd(&s_bytes[2], vp);
+            if (kp) atomicAdd(&s_bytes[3], kp);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (s_bytes[0]) atomicAdd(&st->bytes_batched, s_bytes[0]);
+            if (s_bytes[1]) atomicAdd(&st->bytes_per_view, s_bytes[1]);
+            if (s_bytes[2]) atomicAdd(&st->points_per_view, s_bytes[2]);
+            if (s_bytes[3]) atomicAdd(&st->points_any_view, s_bytes[3]);
+        }
+    }
     const unsigned long long m = __ballot(keep);
     if (m) {
         const uint32_t lane = __lane_id();
@@ -575,6 +613,19 @@ int pcr_multiview_launch(pcr_multiview* mv) {
         return PCR_OK;
     }
     return mv_enqueue(mv, mv->ctx->stream);
+}
+
+int pcr_multiview_stats_get(pcr_multiview* mv, pcr_multiview_stats* out) {
+    PCR_CHECK(mv && out, "multiview_stats: null argument");
+    PCR_HIP(hipStreamSynchronize(mv->ctx->stream));
+    MVState st;
+    PCR_HIP(hipMemcpy(&st, mv->B.d_state, sizeof(st), hipMemcpyDeviceToHost));
+    out->batches_any_view = st.list_count;
+    out->bytes_batched = st.bytes_batched;
+    out->bytes_per_view_sum = st.bytes_per_view;
+    out->points_per_view_sum = st.points_per_view;
+    out->points_any_view = st.points_any_view;
+    return PCR_OK;
 }
 
 int pcr_multiview_destroy(pcr_multiview* mv) {

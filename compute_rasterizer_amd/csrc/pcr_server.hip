@@ -86,6 +86,7 @@ struct Slot {
     float* h_lod = nullptr;
     unsigned long long** h_fbs = nullptr;
     uint32_t** h_out = nullptr;
+    MVState* h_state = nullptr;              // pinned copy of the launch's counters
     std::vector<uint64_t> tickets;
     bool busy = false;
 };
@@ -131,6 +132,7 @@ static int slot_alloc(Slot& s, int max_batch, uint64_t nb) {
         PCR_HIP(hipHostMalloc(&s.h_lod, sizeof(float) * max_batch));
         PCR_HIP(hipHostMalloc(&s.h_fbs, sizeof(void*) * max_batch));
         PCR_HIP(hipHostMalloc(&s.h_out, sizeof(void*) * max_batch));
+        PCR_HIP(hipHostMalloc(&s.h_state, sizeof(MVState)));
     }
     return PCR_OK;
 }
@@ -170,6 +172,7 @@ static int launch_group(pcr_server* srv, Slot& sl, const std::vector<uint64_t>& 
         const Req& q = srv->reqs[group[k]];
         PCR_HIP(hipMemcpyAsync(srv->h_arena + q.hoff, sl.h_out[k], 4 * q.npix, hipMemcpyDeviceToHost, s));
     }
+    PCR_HIP(hipMemcpyAsync(sl.h_state, sl.B.d_state, sizeof(MVState), hipMemcpyDeviceToHost, s));
     PCR_HIP(hipEventRecord(sl.done, s));
     sl.tickets = group;
     sl.busy = true;
@@ -201,7 +204,12 @@ static void worker_main(pcr_server* srv) {
                     srv->st.points_rendered += srv->clouds[r.r.cloud_id]->num_points;
                 }
             }
-            if (q != hipSuccess) srv->worker_error = PCR_E_HIP;
+            if (q != hipSuccess) {
+                srv->worker_error = PCR_E_HIP;
+            } else {   // the counters' copy precedes the event on the slot's stream
+                srv->st.points_drawn += sl.h_state->points_per_view;
+                srv->st.bytes_algorithmic += sl.h_state->bytes_batched;
+            }
             sl.tickets.clear();
             sl.busy = false;
             progress = true;
@@ -456,6 +464,7 @@ int pcr_server_destroy(pcr_server* srv) {
         mv_release(sl.B);
         (void)hipHostFree(sl.h_views); (void)hipHostFree(sl.h_lod); (void)hipHostFree(sl.h_fbs);
         (void)hipHostFree(sl.h_out);
+        (void)hipHostFree(sl.h_state);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }

@@ -166,6 +166,11 @@ typedef struct pcr_stats {
                                       rectangle already held a nearer point (hqs:
                                       both passes, a batch per pass)            */
     uint64_t points_occluded;      /* the points of those batches                 */
+    uint64_t bytes_occluded;       /* the plane bytes (hqs pass 2: and colour
+                                      bytes) those batches did not read; the
+                                      raster's bytes are bytes_algorithmic minus
+                                      these (hqs: twice the planes plus the colour
+                                      of the drawn points, minus these)          */
     uint64_t atomics_issued;       /* 64-bit framebuffer atomicMins issued (LAS,
                                       LAS2, NODES): those whose compare load found
                                       a larger key                                */
@@ -239,6 +244,12 @@ typedef struct pcr_gen_cfg {
     float    jitter;         /* TERRAIN: +-jitter (m)                            */
     float    terrain_frac;   /* CITY: fraction of points on the terrain          */
 } pcr_gen_cfg;
+
+/* Ground height of a TERRAIN or CITY scene at (x, y): the fBm heightfield
+ * the generator puts the terrain points on (before their jitter), evaluated
+ * on the host with the generator's own operations. For cameras that keep a
+ * height above the ground. */
+int pcr_scene_ground_height(const pcr_gen_cfg* cfg, float x, float y, float* height);
 
 /* Generate points on the device: xyz float[3n] + rgba uint32[n], unsorted. */
 int pcr_generate_points(pcr_ctx* ctx, const pcr_gen_cfg* cfg, float* xyz_dev, uint32_t* rgba_dev);
@@ -465,6 +476,18 @@ int pcr_multiview_create(pcr_ctx* ctx, pcr_cloud* cloud, const pcr_view* views, 
                          const pcr_tune* tune, int use_graph, pcr_multiview** out);
 int pcr_multiview_set_views(pcr_multiview* mv, const pcr_view* views, int nviews);
 int pcr_multiview_launch(pcr_multiview* mv);
+/* Counts of the plan's last launch (synchronises). The algorithmic bytes are
+ * the batch headers and position planes the raster needs: for the batched
+ * form each kept batch once, at the most precise level any view needs; for
+ * separate per-view renders the sum over the views that draw it. */
+typedef struct pcr_multiview_stats {
+    uint64_t batches_any_view;     /* batches drawn by at least one view        */
+    uint64_t bytes_batched;        /* 64 + 4 x planes(most precise level) x n    */
+    uint64_t bytes_per_view_sum;   /* sum over views of 64 + 4 x planes(level) x n */
+    uint64_t points_per_view_sum;  /* points in drawn batches, summed over views */
+    uint64_t points_any_view;      /* points in batches drawn by at least one view */
+} pcr_multiview_stats;
+int pcr_multiview_stats_get(pcr_multiview* mv, pcr_multiview_stats* out);
 int pcr_multiview_destroy(pcr_multiview* mv);
 /* Sweep block size / points per thread (untiled) and block size / tile size
  * (LDS-tiled) for this cloud, views and mode; writes the fastest
@@ -490,6 +513,12 @@ typedef struct pcr_server_stats {
     uint64_t points_rendered;   /* sum over completed requests of cloud points */
     uint64_t fb_bytes_in_use;
     uint64_t fb_bytes_peak;
+    uint64_t points_drawn;      /* points in the batches or LOD nodes the
+                                   completed launches drew, summed over their
+                                   requests                                   */
+    uint64_t bytes_algorithmic; /* headers + planes those launches read (each
+                                   batch once per launch, at the most precise
+                                   level its requests need)                   */
 } pcr_server_stats;
 
 int pcr_server_create(pcr_ctx* ctx, uint64_t fb_budget_bytes, int max_batch, int num_streams,

@@ -65,6 +65,8 @@ def load():
         f.orc_lod_select.argtypes = [P, C.c_uint32, C.POINTER(View), C.c_int, C.c_int, C.c_float, P]
         f.orc_render_nodes.argtypes = [P, C.c_uint32, P, P, P, C.POINTER(View), C.c_int, C.c_int, C.c_float, P, P]
         f.orc_render_batched_mt.argtypes = [P, C.c_uint32, P, P, P, C.POINTER(View), C.c_int, P, C.c_int]
+        f.orc_render_nodes_mt.argtypes = [P, C.c_uint32, P, P, P, C.POINTER(View), C.c_int, C.c_int, C.c_float, P,
+                                          C.c_int]
         f.orc_las_parse_header.restype = C.c_int
         f.orc_las_parse_header.argtypes = [P, C.c_uint64, C.POINTER(LasInfo)]
         f.orc_las_encode.argtypes = [P, C.c_uint64, C.POINTER(LasInfo), C.c_uint32, C.c_uint64, C.c_uint32,
@@ -249,6 +251,13 @@ def render_nodes(nodes, lo, med, hi, view: View, culling=1, rule=0, lod_px=1.0):
     load().orc_render_nodes(_p(nodes), len(nodes), _p(lo), _p(med), _p(hi), C.byref(view), culling, rule, lod_px,
                             _p(fb), _p(flags))
     return fb, flags
+
+
+def render_nodes_mt(nodes, lo, med, hi, view: View, culling=1, rule=0, lod_px=1.0, nthreads=8):
+    fb = new_fb(view)
+    load().orc_render_nodes_mt(_p(nodes), len(nodes), _p(lo), _p(med), _p(hi), C.byref(view), culling, rule,
+                               lod_px, _p(fb), nthreads)
+    return fb
 
 
 def look_at(eye, target, up=(0, 1, 0), fovy=60.0, near=0.1, far=1000.0, origin=(0, 0, 0), width=1280,

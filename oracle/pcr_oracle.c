@@ -913,6 +913,41 @@ void orc_render_batched_mt(const pcr_batch* b, uint32_t nb, const uint32_t* lo, 
     free(th);
 }
 
+/* orc_render_nodes with nthreads threads over the drawn nodes (CAS fetch-min;
+ * min is order independent, so the result equals the single-thread one) */
+typedef struct {
+    const pcr_batch* nodes; uint32_t nn; const uint32_t *lo, *med, *hi; const pcr_view* v;
+    const uint8_t* flags; uint64_t* fb; uint32_t* next;
+} nodes_job;
+static void* nodes_worker(void* arg) {
+    nodes_job* j = (nodes_job*)arg;
+    for (;;) {
+        uint32_t k0 = __atomic_fetch_add(j->next, 8u, __ATOMIC_RELAXED);
+        if (k0 >= j->nn) break;
+        uint32_t k1 = k0 + 8u < j->nn ? k0 + 8u : j->nn;
+        for (uint32_t k = k0; k < k1; k++) {
+            if (!j->flags[k]) continue;
+            int level = orc_precision_level(j->nodes[k].min, j->nodes[k].max, j->v, 80.0f);
+            render_batch(&j->nodes[k], j->lo, j->med, j->hi, j->v, level, j->fb, 1);
+        }
+    }
+    return NULL;
+}
+void orc_render_nodes_mt(const pcr_batch* nodes, uint32_t nn, const uint32_t* lo, const uint32_t* med,
+                         const uint32_t* hi, const pcr_view* v, int culling, int rule, float lod_px, uint64_t* fb,
+                         int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    uint8_t* flags = (uint8_t*)malloc(nn ? nn : 1);
+    orc_lod_select(nodes, nn, v, culling, rule, lod_px, flags);
+    uint32_t next = 0;
+    nodes_job job = {nodes, nn, lo, med, hi, v, flags, fb, &next};
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, nodes_worker, &job);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(flags);
+}
+
 /* ---------------------------------------------------------------- camera */
 static void mat_mul_d(const double a[16], const double b[16], double r[16]) {
     for (int c = 0; c < 4; c++)

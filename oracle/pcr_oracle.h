@@ -114,6 +114,9 @@ void orc_render_nodes(const pcr_batch* nodes, uint32_t nn, const uint32_t* lo, c
 void orc_render_batched_mt(const pcr_batch* b, uint32_t nb, const uint32_t* lo, const uint32_t* med,
                            const uint32_t* hi, const pcr_view* v, int frustum_culling,
                            uint64_t* fb, int nthreads);
+void orc_render_nodes_mt(const pcr_batch* nodes, uint32_t nn, const uint32_t* lo, const uint32_t* med,
+                         const uint32_t* hi, const pcr_view* v, int culling, int rule, float lod_px, uint64_t* fb,
+                         int nthreads);
 
 /* ---- host camera (double precision, glm conventions) ------------------- */
 void orc_look_at_view(const double eye[3], const double target[3], const double up[3],

@@ -40,7 +40,7 @@ MIRRORS = {"Batch": "pcr_batch", "View": "pcr_view", "RenderParams": "pcr_render
            "GenCfg": "pcr_gen_cfg", "BuildOpts": "pcr_build_opts", "CloudInfo": "pcr_cloud_info",
            "LasInfo": "pcr_las_info", "LasFile": "pcr_las_file", "PotreeInfo": "pcr_potree_info",
            "OctreeOpts": "pcr_octree_opts", "Tune": "pcr_tune", "Request": "pcr_request",
-           "ServerStats": "pcr_server_stats"}
+           "ServerStats": "pcr_server_stats", "MultiViewStats": "pcr_multiview_stats"}
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -124,3 +124,20 @@ def test_host_encoder_matches_oracle(n, bs):
 def test_host_encoder_rejects_unaligned_batch_size():
     with pytest.raises(pcr.PcrError):
         pcr.encode_batches_host(np.zeros((8, 3), np.float32), 10)
+
+
+def test_ground_height_matches_oracle_generator():
+    """The host ground height is the generator's heightfield: equal to the
+    oracle's restatement bit for bit, and every generated terrain point lies
+    within its jitter of it."""
+    cfg = pcr.gen_cfg(pcr.SCENE_CITY, 10_000, seed=42, extent=2000.0, amplitude=150.0, jitter=0.5)
+    rng = np.random.default_rng(0)
+    for x, y in rng.uniform(0, 2000, (200, 2)):
+        want = orc.load().orc_terrain_height(42, 2000.0, 150.0, float(np.float32(x)), float(np.float32(y)))
+        assert pcr.ground_height(cfg, float(np.float32(x)), float(np.float32(y))) == want
+    tcfg = pcr.gen_cfg(pcr.SCENE_TERRAIN, 5000, seed=3, jitter=0.5)
+    xyz, _ = orc.generate(tcfg)
+    g = np.array([pcr.ground_height(tcfg, float(p[0]), float(p[1])) for p in xyz[:500]])
+    assert np.all(np.abs(xyz[:500, 2] - g) <= 1.0 + 1e-3)
+    with pytest.raises(pcr.PcrError):
+        pcr.ground_height(pcr.gen_cfg(pcr.SCENE_CUBE, 10), 0.0, 0.0)

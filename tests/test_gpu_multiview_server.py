@@ -131,13 +131,35 @@ def test_server_same_result_alone_and_in_mix(ctx, city, nodes):
     mixed = [srv.wait(t, v.width * v.height)[0] for t, (_, _, v) in zip(tickets, reqs)]
     for a, b in zip(alone, mixed):
         assert np.array_equal(a, b)
-    # and both equal the plain single-view render
+    # both equal the plain single-view render and the oracle
     cid, mode, v = reqs[0]
     _, img = single_view(ctx, city if cid == c0 else nodes, v, pcr.params(mode, lod_px=1.0))
     assert np.array_equal(alone[0], img)
+    lays = {c0: city.download(), c1: nodes.download()}
+    for (cid, mode, v), img in zip(reqs, mixed):
+        L = lays[cid]
+        if cid == c0:
+            ref, _ = orc.render_batched(L["batches"], L["lo"], L["med"], L["hi"], v, 1)
+        else:
+            ref, _ = orc.render_nodes(L["batches"], L["lo"], L["med"], L["hi"], v, 1, pcr.LOD_SPACING, 1.0)
+        assert np.array_equal(img, orc.resolve_index(ref, L["rgba"], pcr.BG_LAS))
     st = srv.stats()
     assert st.completed == 2 * len(reqs)
     assert st.fb_bytes_peak <= 3 * big and st.fb_bytes_in_use == 0
+    srv.close()
+
+
+def test_server_image_size_is_the_tickets(ctx, city):
+    srv = pcr.Server(ctx, budget_bytes=12 * 640 * 480 * 2, max_batch=2, num_streams=1)
+    cid = srv.add_cloud(city)
+    v = views_for([(640, 480)], seed=3)[0]
+    t = srv.submit(cid, v)
+    with pytest.raises(ValueError):
+        srv.wait(t, 100)                 # the image has 640 x 480 pixels: a shorter buffer is refused
+    img, _ = srv.wait(t)
+    assert img.shape == (640 * 480,)
+    with pytest.raises(pcr.PcrError):
+        srv.wait(t)                      # taken
     srv.close()
 
 

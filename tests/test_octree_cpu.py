@@ -103,3 +103,16 @@ def test_rejects_non_power_of_two_spacing_div():
     xyz, ids = cloud(1000)
     with pytest.raises(pcr.PcrError):
         pcr.octree_layout_host(xyz, ids, root_spacing_div=100.0)
+
+
+def test_nodes_mt_equals_single_thread():
+    """The oracle's multithreaded LOD renderer (CAS fetch-min over the drawn
+    nodes) gives the single-thread frame for both LOD rules."""
+    xyz, rgba = orc.generate(pcr.gen_cfg(pcr.SCENE_TERRAIN, 200_000, seed=5))
+    nodes, lo, med, hi, rgb = pcr.octree_layout_host(xyz, rgba, seed=3, max_node_points=4000)
+    v = pcr.look_at((1000, -300, 400), (1000, 1000, 30), up=(0, 0, 1), near=1, far=5000, width=320, height=200)
+    for rule in (pcr.LOD_SPACING, pcr.LOD_SIZE80):
+        ref, _ = orc.render_nodes(nodes, lo, med, hi, v, 1, rule, 1.0)
+        assert (ref != pcr.FB_CLEAR).sum() > 1000
+        for threads in (1, 4, 7):
+            assert np.array_equal(orc.render_nodes_mt(nodes, lo, med, hi, v, 1, rule, 1.0, threads), ref)
