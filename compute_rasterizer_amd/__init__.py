@@ -70,6 +70,26 @@ def potree_info(metadata_json: str) -> capi.PotreeInfo:
     return info
 
 
+def laz_info(data) -> capi.LazInfo:
+    """Header, LASzip VLR and chunk table of LAZ file bytes (host only)."""
+    b = np.frombuffer(data, np.uint8)
+    out = capi.LazInfo()
+    check(capi.load().pcr_laz_info_get(b.ctypes.data, b.size, C.byref(out)))
+    return out
+
+
+def laz_decode(data, first=0, count=None, fields=capi.LAZ_ALL_FIELDS, threads=0) -> np.ndarray:
+    """Records [first, first + count) of LAZ file bytes as a (count, record_length)
+    uint8 array: the uncompressed LAS records (host only)."""
+    b = np.frombuffer(data, np.uint8)
+    info = laz_info(data)
+    n = int(info.num_points) - first if count is None else count
+    out = np.empty((max(n, 0), info.record_length), np.uint8)
+    check(capi.load().pcr_laz_decode(b.ctypes.data, b.size, first, n, out.ctypes.data if n else None, fields,
+                                     threads))
+    return out
+
+
 def gen_cfg(scene, n, seed=1, extent=2000.0, amplitude=150.0, jitter=0.5, num_buildings=256,
             terrain_frac=0.2) -> GenCfg:
     return GenCfg(scene, num_buildings, seed, n, extent, amplitude, jitter, terrain_frac)

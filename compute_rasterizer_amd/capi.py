@@ -68,8 +68,21 @@ class RenderParams(C.Structure):
 class LasInfo(C.Structure):
     _fields_ = [("num_points", C.c_uint64), ("offset_to_points", C.c_uint32), ("record_length", C.c_uint16),
                 ("point_format", C.c_uint8), ("version_major", C.c_uint8), ("version_minor", C.c_uint8),
-                ("rgb_offset", C.c_uint8), ("pad", C.c_uint8 * 6), ("scale", C.c_double * 3),
+                ("rgb_offset", C.c_uint8), ("compressed", C.c_uint8), ("pad", C.c_uint8 * 5),
+                ("scale", C.c_double * 3),
                 ("offset", C.c_double * 3), ("min", C.c_double * 3), ("max", C.c_double * 3)]
+
+
+class LazInfo(C.Structure):
+    _fields_ = [("num_points", C.c_uint64), ("record_length", C.c_uint32), ("num_chunks", C.c_uint32),
+                ("chunk_size", C.c_uint32), ("compressor", C.c_uint16), ("num_items", C.c_uint16),
+                ("item_type", C.c_uint16 * 8), ("item_size", C.c_uint16 * 8), ("item_version", C.c_uint16 * 8),
+                ("point_format", C.c_uint8), ("has_rgb", C.c_uint8), ("rgb_offset", C.c_uint8),
+                ("pad", C.c_uint8 * 5)]
+
+
+LAZ_ALL_FIELDS = 0
+LAZ_XYZ_RGB = 1
 
 
 class PotreeInfo(C.Structure):
@@ -178,6 +191,8 @@ SIGNATURES = {
     "pcr_render_static": (C.c_int, [P, u32, P, P, P, u64, C.POINTER(View), C.POINTER(RenderParams), P, P]),
     "pcr_stats_get": (C.c_int, [P, C.POINTER(Stats)]),
     "pcr_las_parse_header": (C.c_int, [P, C.c_uint64, P]),
+    "pcr_laz_info_get": (C.c_int, [P, C.c_uint64, P]),
+    "pcr_laz_decode": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, P, C.c_uint32, C.c_int]),
     "pcr_cloud_from_las": (C.c_int, [P, P, C.c_int, P, C.POINTER(C.c_void_p)]),
     "pcr_cloud_from_potree": (C.c_int, [P, P, P, C.c_uint64, P, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pcr_frame_times": (C.c_int, [P, C.c_int, P, C.POINTER(C.c_int)]),

@@ -168,6 +168,10 @@ int launch_mv_raster_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B
 int launch_mv_hqs_tiled(hipStream_t s, const pcr_cloud* c, const MVBuffers& B, int nviews, int tile_px, int block,
                         int cu_count, bool occl = true);
 
+// host parsers (pcr_parse.cpp)
+int potree_layout(const pcr_potree_info* info, const void* hierarchy, uint64_t hierarchy_size, uint64_t octree_size,
+                  std::vector<pcr_batch>& hb, std::vector<double>& boxes, int* max_level);
+
 // layout (pcr_layout.hip)
 int generate_points(pcr_ctx* ctx, const pcr_gen_cfg* cfg, float* xyz, uint32_t* rgba);
 int build_cloud_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, uint64_t n,

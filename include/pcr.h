@@ -300,7 +300,10 @@ typedef struct pcr_las_info {
     uint8_t  rgb_offset;        /* R,G,B u16 offset in a record: 20 (format 2),
                                    28 (3), 30 (7, 8), else 0 (LasLoaderSparse.cpp:
                                    123-132 reads the first six bytes then)       */
-    uint8_t  pad[6];
+    uint8_t  compressed;        /* 1: the point data is LAZ (LASzip VLR present,
+                                   @104 bit 7 or 6 set); the record size check
+                                   is then left to pcr_laz_info_get             */
+    uint8_t  pad[5];
     double   scale[3];          /* @131                                           */
     double   offset[3];         /* @155                                           */
     double   min[3];            /* @187, @203, @219                               */
@@ -314,8 +317,54 @@ typedef struct pcr_las_file {
 
 /* Parse a LAS 1.0-1.4 header (host only; no device work). */
 int pcr_las_parse_header(const void* bytes, uint64_t size, pcr_las_info* out);
-/* One cloud from nfiles uncompressed LAS files, encoded on the device as
- * LasLoaderSparse::loadLas does (LasLoaderSparse.cpp:37-212): file k's points
+
+/* ---- LAZ (LASzip-compressed LAS) ---------------------------------------- */
+/* The reference reads .laz through its vendored LASzip (libs/laszip):
+ * laszip_open_reader + laszip_seek_point + laszip_read_point per point
+ * (LasLoaderSparse.cpp:246-300). These replace that reader with this
+ * library's own decoder of the format: compressor 2 (pointwise chunked,
+ * items POINT10 / GPSTIME11 / RGB12 / BYTE version 2: point formats 0-3
+ * with extra bytes) and compressor 3 (layered chunked, items POINT14 / RGB14 /
+ * RGBNIR14 / BYTE14 version 3 or 4: formats 6-8 with extra bytes). Waveform
+ * items (formats 4, 5, 9, 10) and the LASzip 1.x pointwise coder are refused. */
+typedef struct pcr_laz_info {
+    uint64_t num_points;        /* header count (legacy or LAS 1.4 field)         */
+    uint32_t record_length;     /* bytes per decoded record (= the header's)      */
+    uint32_t num_chunks;        /* chunks in the chunk table (0: none, decoded in
+                                   order on one thread)                           */
+    uint32_t chunk_size;        /* points per chunk, 0xFFFFFFFF = variable        */
+    uint16_t compressor;        /* 2 pointwise chunked, 3 layered chunked         */
+    uint16_t num_items;
+    uint16_t item_type[8];      /* LASzip item types: 6 POINT10, 7 GPSTIME11,
+                                   8 RGB12, 0 BYTE, 10 POINT14, 11 RGB14,
+                                   12 RGBNIR14, 14 BYTE14                         */
+    uint16_t item_size[8];
+    uint16_t item_version[8];
+    uint8_t  point_format;      /* @104 with the compression bits cleared         */
+    uint8_t  has_rgb;           /* the records carry R,G,B                        */
+    uint8_t  rgb_offset;        /* their byte offset in a record (0 if none)      */
+    uint8_t  pad[5];
+} pcr_laz_info;
+
+#define PCR_LAZ_ALL_FIELDS 0u   /* decode every field of every record            */
+#define PCR_LAZ_XYZ_RGB    1u   /* layered files: decode only the X/Y, Z and RGB
+                                   layers; the other fields of a record keep the
+                                   value of its chunk's first record (pointwise
+                                   files decode everything either way)           */
+
+/* Header, LASzip VLR and chunk table of a LAZ file (host only). */
+int pcr_laz_info_get(const void* bytes, uint64_t size, pcr_laz_info* out);
+/* Decode records [first_point, first_point + num_points) of a LAZ file into
+ * `records` (num_points * record_length bytes, the uncompressed LAS records)
+ * on `threads` host threads (<= 0: up to 16), one chunk per thread at a time.
+ * A damaged chunk fails the call with PCR_E_INVALID. Host only. */
+int pcr_laz_decode(const void* bytes, uint64_t size, uint64_t first_point, uint64_t num_points, void* records,
+                   uint32_t fields, int threads);
+/* One cloud from nfiles LAS or LAZ files, encoded on the device as
+ * LasLoaderSparse::loadLas does (LasLoaderSparse.cpp:37-212; LAZ files are
+ * first decoded on the host with pcr_laz_decode, and then follow loadLaz,
+ * :214-330, which differs only in colour: a LAZ file without R,G,B gets
+ * colour 0, where loadLas reads the record's first six bytes): file k's points
  * follow file k-1's in the planes (the sparse offset is the running point
  * count), its batches cover 10,240 consecutive points each, positions are
  * X*scale + offset - header min in double, each batch AABB and 30-bit

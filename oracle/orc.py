@@ -276,8 +276,13 @@ def las_parse_header(data: bytes):
     return info if rc == 0 else None
 
 
-def las_encode(data: bytes, info: LasInfo, batch_size=10240, sparse_offset=0, file_index=0):
-    """(batches, lo, med, hi, rgba) of one LAS file, as LasLoaderSparse::loadLas builds them."""
+def las_encode(data: bytes, info: LasInfo, batch_size=10240, sparse_offset=0, file_index=0, laz=False):
+    """(batches, lo, med, hi, rgba) of one LAS file, as LasLoaderSparse::loadLas builds them.
+
+    laz=True: the records came from a LAZ file, which LasLoaderSparse::loadLaz
+    (LasLoaderSparse.cpp:214-330) encodes the same way except for colour: it
+    takes R, G, B from the LASzip point (:283-285), which holds zeros for point
+    formats without colour, where loadLas reads the record's first six bytes."""
     buf = np.frombuffer(data, np.uint8)
     n = int(info.num_points)
     rec = np.ascontiguousarray(buf[info.offset_to_points:info.offset_to_points + n * info.record_length])
@@ -286,6 +291,8 @@ def las_encode(data: bytes, info: LasInfo, batch_size=10240, sparse_offset=0, fi
     lo, med, hi, rgba = (np.zeros(n, np.uint32) for _ in range(4))
     load().orc_las_encode(_p(rec), n, C.byref(info), batch_size, sparse_offset, file_index, _p(b), _p(lo), _p(med),
                           _p(hi), _p(rgba))
+    if laz and info.point_format not in (2, 3, 5, 7, 8, 10):
+        rgba[:] = 0
     return b, lo, med, hi, rgba
 
 

@@ -40,7 +40,8 @@ MIRRORS = {"Batch": "pcr_batch", "View": "pcr_view", "RenderParams": "pcr_render
            "GenCfg": "pcr_gen_cfg", "BuildOpts": "pcr_build_opts", "CloudInfo": "pcr_cloud_info",
            "LasInfo": "pcr_las_info", "LasFile": "pcr_las_file", "PotreeInfo": "pcr_potree_info",
            "OctreeOpts": "pcr_octree_opts", "Tune": "pcr_tune", "Request": "pcr_request",
-           "ServerStats": "pcr_server_stats", "MultiViewStats": "pcr_multiview_stats"}
+           "ServerStats": "pcr_server_stats", "MultiViewStats": "pcr_multiview_stats",
+           "LazInfo": "pcr_laz_info"}
 
 
 def test_struct_layouts_match_header(tmp_path):
