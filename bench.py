@@ -22,6 +22,7 @@ import json
 import math
 import os
 import platform
+import struct
 import sys
 import time
 
@@ -31,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import compute_rasterizer_amd as pcr  # noqa: E402
+from compute_rasterizer_amd import capi  # noqa: E402
 
 # HBM bytes per launch of the default raster kernel on the default workload,
 # from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/
@@ -533,6 +535,54 @@ def serving_leg(ctx, seconds, rates, budget_gb, sizes, seed=11):
             "poisson": phases, "burst": burst}
 
 
+def laz_ingest(ctx, n_points):
+    """LAZ ingest: two LAZ files of about n_points / 2 points each (point
+    formats 3 and 7, made by repeating the chunks of two LAZ fixtures that the
+    reference's LASzip wrote, tests/golden/laz/), decoded on the host threads
+    and encoded on the device; against the same records as uncompressed LAS."""
+    from compute_rasterizer_amd.datasets import laz_tile
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
+    store = np.load(os.path.join(golden, "laz_records.npz"))
+    lazs, lases = [], []
+    for name, key in (("f3_extra3_variable", "f3_extra3_variable"), ("f7_v4", "f7_v3")):
+        with open(os.path.join(golden, "laz", name + ".laz"), "rb") as f:
+            seed = f.read()
+        rec = store[key + "_records"]
+        copies = max(1, int(n_points // 2 // rec.shape[0]))
+        lazs.append(laz_tile(seed, copies))
+        hdr = bytearray(store[key + "_las_header"].tobytes())
+        n = copies * rec.shape[0]
+        struct.pack_into("<I", hdr, 107, n)
+        if len(hdr) >= 255:
+            struct.pack_into("<Q", hdr, 247, n)
+        lases.append(bytes(hdr) + np.tile(rec, (copies, 1)).tobytes())
+    npts = sum(int(pcr.laz_info(d).num_points) for d in lazs)
+    res = {"files": "formats 3 (pointwise, extra bytes) and 7 (layered v4)", "points": npts,
+           "laz_bytes": sum(len(d) for d in lazs), "las_bytes": sum(len(d) for d in lases),
+           "host_threads": min(16, os.cpu_count() or 1)}
+    for fields, key in ((capi.LAZ_ALL_FIELDS, "decode_all_fields"), (capi.LAZ_XYZ_RGB, "decode_xyz_rgb")):
+        t = time.perf_counter()
+        for d in lazs:
+            pcr.laz_decode(d, fields=fields)
+        dt = time.perf_counter() - t
+        res[key + "_Mpoints_s"] = round(npts / dt / 1e6, 1)
+    t = time.perf_counter()
+    a, _ = ctx.from_las(lazs)
+    dt = time.perf_counter() - t
+    res["laz_to_cloud_s"] = round(dt, 3)
+    res["laz_to_cloud_Mpoints_s"] = round(npts / dt / 1e6, 1)
+    t = time.perf_counter()
+    b, _ = ctx.from_las(lases)
+    dt = time.perf_counter() - t
+    res["las_to_cloud_Mpoints_s"] = round(npts / dt / 1e6, 1)
+    la, lb = a.download(), b.download()
+    res["bit_exact_vs_las"] = bool(all(np.array_equal(la[k], lb[k]) for k in ("lo", "med", "hi", "rgba")) and
+                                   np.array_equal(la["batches"].view(np.uint8), lb["batches"].view(np.uint8)))
+    a.close()
+    b.close()
+    return res
+
+
 def ingest_leg(ctx, n_las, seed=5):
     """LAS and Potree 2.0 ingest (host bytes -> device cloud, including the
     host-to-device copy of the records), and LOD frames of the Potree cloud
@@ -552,6 +602,7 @@ def ingest_leg(ctx, n_las, seed=5):
                   "Mpoints_s": round(n / las_s / 1e6, 1), "batches": int(first[-1])}
     cloud.close()
     del files
+    out["laz"] = laz_ingest(ctx, n_las // 2)
     meta, hier, octree, _ = make_potree(seed=seed, max_depth=5, points_per_node=(2000, 8000))
     t = time.perf_counter()
     pc = ctx.from_potree(meta, hier, octree)

@@ -78,6 +78,19 @@ def laz_info(data) -> capi.LazInfo:
     return out
 
 
+def laz_chunk_table(data):
+    """(chunk_start, chunk_first) of a LAZ file: chunk c holds records
+    [chunk_first[c], chunk_first[c+1]) in bytes [chunk_start[c], chunk_start[c+1])."""
+    b = np.frombuffer(data, np.uint8)
+    n = C.c_uint32()
+    check(capi.load().pcr_laz_chunk_table(b.ctypes.data, b.size, None, None, 0, C.byref(n)))
+    starts, firsts = np.zeros(n.value + 1, np.uint64), np.zeros(n.value + 1, np.uint64)
+    if n.value:
+        check(capi.load().pcr_laz_chunk_table(b.ctypes.data, b.size, ptr(starts), ptr(firsts), n.value + 1,
+                                              C.byref(n)))
+    return starts, firsts
+
+
 def laz_decode(data, first=0, count=None, fields=capi.LAZ_ALL_FIELDS, threads=0) -> np.ndarray:
     """Records [first, first + count) of LAZ file bytes as a (count, record_length)
     uint8 array: the uncompressed LAS records (host only)."""

@@ -192,6 +192,7 @@ SIGNATURES = {
     "pcr_stats_get": (C.c_int, [P, C.POINTER(Stats)]),
     "pcr_las_parse_header": (C.c_int, [P, C.c_uint64, P]),
     "pcr_laz_info_get": (C.c_int, [P, C.c_uint64, P]),
+    "pcr_laz_chunk_table": (C.c_int, [P, C.c_uint64, P, P, C.c_uint32, P]),
     "pcr_laz_decode": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, P, C.c_uint32, C.c_int]),
     "pcr_cloud_from_las": (C.c_int, [P, P, C.c_int, P, C.POINTER(C.c_void_p)]),
     "pcr_cloud_from_potree": (C.c_int, [P, P, P, C.c_uint64, P, C.c_uint64, C.POINTER(C.c_void_p)]),

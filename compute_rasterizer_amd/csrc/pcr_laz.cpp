@@ -1321,6 +1321,29 @@ int pcr_laz_info_get(const void* bytes, uint64_t size, pcr_laz_info* out) {
     return PCR_OK;
 }
 
+int pcr_laz_chunk_table(const void* bytes, uint64_t size, uint64_t* chunk_start, uint64_t* chunk_first,
+                        uint32_t capacity, uint32_t* num_chunks) {
+    PCR_CHECK(bytes && num_chunks, "laz_chunk_table: null argument");
+    laz::File f;
+    try {
+        const int rc = laz::open_file(static_cast<const uint8_t*>(bytes), size, f);
+        if (rc) return rc;
+    } catch (const std::exception& e) {
+        set_error("laz_chunk_table: %s", e.what());
+        return PCR_E_INVALID;
+    }
+    const uint32_t n = f.chunk_start.empty() ? 0 : (uint32_t)(f.chunk_start.size() - 1);
+    *num_chunks = n;
+    if (n == 0 || (capacity == 0 && !chunk_start && !chunk_first)) return PCR_OK;   // the count only
+    PCR_CHECK(capacity >= n + 1 && chunk_start && chunk_first, "laz_chunk_table: %u chunks need %u entries, got %u",
+              n, n + 1, capacity);
+    for (uint32_t c = 0; c <= n; c++) {
+        chunk_start[c] = f.chunk_start[c];
+        chunk_first[c] = std::min(f.chunk_first[c], f.num_points);
+    }
+    return PCR_OK;
+}
+
 static int laz_decode(const void* bytes, uint64_t size, uint64_t first_point, uint64_t num_points, void* records,
                       uint32_t fields, int threads) {
     PCR_CHECK(bytes && (records || num_points == 0), "laz_decode: null argument");

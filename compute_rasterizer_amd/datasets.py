@@ -62,6 +62,189 @@ def make_las(n, point_format=2, version=(1, 2), seed=0, scale=(0.001, 0.001, 0.0
     return bytes(h) + rec.tobytes(), X, Y, Z, R, G, B
 
 
+# ---------------------------------------------------------------- LAZ tiling
+# A large LAZ file for throughput runs, made from a small one by repeating its
+# chunks (each chunk restarts every model, so a chunk decodes the same
+# anywhere) behind a new chunk table. The table is arithmetic coded: an
+# integer compressor (32 bits, 2 contexts) codes each chunk's point count
+# (context 0) and byte size (context 1) as a difference from the previous
+# chunk's. The encoder below is the inverse of the decoder in
+# compute_rasterizer_amd/csrc/pcr_laz.cpp (A. Said's FastAC coder as LASzip
+# uses it).
+
+class _Model:
+    def __init__(self, n):
+        self.n, self.cnt, self.total, self.cycle = n, [1] * n, 0, n
+        self.dist = [0] * n
+        self._update()
+        self.cycle = self.until = (n + 6) >> 1
+
+    def _update(self):
+        self.total += self.cycle
+        if self.total > (1 << 15):
+            self.cnt = [(c + 1) >> 1 for c in self.cnt]
+            self.total = sum(self.cnt)
+        scale, acc = 0x80000000 // self.total, 0
+        for k in range(self.n):
+            self.dist[k] = ((scale * acc) & 0xFFFFFFFF) >> 16
+            acc += self.cnt[k]
+        self.cycle = min((5 * self.cycle) >> 2, (self.n + 6) << 3)
+        self.until = self.cycle
+
+
+class _BitModel:
+    def __init__(self):
+        self.p0, self.n0, self.n, self.cycle, self.until = 1 << 12, 1, 2, 4, 4
+
+    def _update(self):
+        self.n += self.cycle
+        if self.n > (1 << 13):
+            self.n, self.n0 = (self.n + 1) >> 1, (self.n0 + 1) >> 1
+            if self.n0 == self.n:
+                self.n += 1
+        self.p0 = (self.n0 * (0x80000000 // self.n)) >> 18
+        self.cycle = min((5 * self.cycle) >> 2, 64)
+        self.until = self.cycle
+
+
+class _Encoder:
+    MIN = 1 << 24
+
+    def __init__(self):
+        self.base, self.length, self.out = 0, 0xFFFFFFFF, bytearray()
+
+    def _carry(self):
+        i = len(self.out) - 1
+        while self.out[i] == 0xFF:
+            self.out[i] = 0
+            i -= 1
+        self.out[i] += 1
+
+    def _step(self, x, length):
+        init = self.base
+        self.base = (self.base + x) & 0xFFFFFFFF
+        self.length = length
+        if init > self.base:
+            self._carry()
+        while self.length < self.MIN:
+            self.out.append(self.base >> 24)
+            self.base = (self.base << 8) & 0xFFFFFFFF
+            self.length = (self.length << 8) & 0xFFFFFFFF
+
+    def symbol(self, m, s):
+        if s == m.n - 1:
+            x = m.dist[s] * (self.length >> 15)
+            self._step(x, self.length - x)
+        else:
+            ln = self.length >> 15
+            x = m.dist[s] * ln
+            self._step(x, m.dist[s + 1] * ln - x)
+        m.cnt[s] += 1
+        m.until -= 1
+        if m.until == 0:
+            m._update()
+
+    def bit(self, m, b):
+        x = m.p0 * (self.length >> 13)
+        if b == 0:
+            m.n0 += 1
+            self._step(0, x)
+        else:
+            self._step(x, self.length - x)
+        m.until -= 1
+        if m.until == 0:
+            m._update()
+
+    def bits(self, n, v):
+        if n > 19:
+            self.bits(16, v & 0xFFFF)
+            v, n = v >> 16, n - 16
+        ln = self.length >> n
+        self._step(v * ln, ln)
+
+    def done(self):
+        init = self.base
+        another = self.length > 2 * self.MIN
+        if another:
+            self.base, self.length = (self.base + self.MIN) & 0xFFFFFFFF, self.MIN >> 1
+        else:
+            self.base, self.length = (self.base + (self.MIN >> 1)) & 0xFFFFFFFF, self.MIN >> 9
+        if init > self.base:
+            self._carry()
+        while self.length < self.MIN:
+            self.out.append(self.base >> 24)
+            self.base = (self.base << 8) & 0xFFFFFFFF
+            self.length = (self.length << 8) & 0xFFFFFFFF
+        self.out += b"\0\0\0" if another else b"\0\0"
+        return bytes(self.out)
+
+
+def _encode_chunk_table(counts, sizes):
+    """The arithmetic-coded body of a LAZ chunk table (variable chunk sizes)."""
+    enc = _Encoder()
+    k_models = [_Model(33), _Model(33)]
+    c0 = _BitModel()
+    corr = [None] + [_Model(1 << min(i, 8)) for i in range(1, 33)]
+
+    def put(pred, real, ctx):
+        c = (real - pred) & 0xFFFFFFFF
+        c = c - (1 << 32) if c >= 1 << 31 else c
+        c1 = -c if c <= 0 else c - 1
+        k = c1.bit_length()
+        enc.symbol(k_models[ctx], k)
+        if k == 0:
+            enc.bit(c0, c)
+        elif k < 32:
+            c = c + (1 << k) - 1 if c < 0 else c - 1
+            if k <= 8:
+                enc.symbol(corr[k], c)
+            else:
+                low = k - 8
+                enc.symbol(corr[k], c >> low)
+                enc.bits(low, c & ((1 << low) - 1))
+
+    pc = pb = 0
+    for n, b in zip(counts, sizes):
+        put(pc, int(n), 0)
+        put(pb, int(b), 1)
+        pc, pb = int(n), int(b)
+    return enc.done()
+
+
+def laz_tile(data: bytes, copies: int) -> bytes:
+    """A LAZ file whose records are `copies` repetitions of data's records: its
+    chunks repeated behind a new (variable-size) chunk table."""
+    from . import laz_chunk_table
+    starts, firsts = laz_chunk_table(data)
+    assert len(starts) > 1, "the file has no chunk table"
+    n = int(firsts[-1])
+    p0 = struct.unpack_from("<I", data, 96)[0]
+    chunks = data[int(starts[0]):int(starts[-1])]
+    counts = np.diff(firsts).astype(np.int64)
+    sizes = np.diff(starts).astype(np.int64)
+    head = bytearray(data[:p0])
+    total = n * copies
+    if head[24] == 1 and head[25] < 4:
+        assert total < 2**32
+        struct.pack_into("<I", head, 107, total)
+    else:
+        struct.pack_into("<I", head, 107, total if total < 2**32 else 0)
+        struct.pack_into("<Q", head, 247, total)
+    # the LASzip VLR's chunk size becomes "variable"
+    hs, nvlr = struct.unpack_from("<H", head, 94)[0], struct.unpack_from("<I", head, 100)[0]
+    p = hs
+    for _ in range(nvlr):
+        rec, ln = struct.unpack_from("<HH", head, p + 18)
+        if rec == 22204 and head[p + 2:p + 16] == b"laszip encoded":
+            struct.pack_into("<I", head, p + 54 + 12, 0xFFFFFFFF)
+        p += 54 + ln
+    body = chunks * copies
+    table_at = p0 + 8 + len(body)
+    table = struct.pack("<II", 0, len(counts) * copies) + _encode_chunk_table(np.tile(counts, copies),
+                                                                             np.tile(sizes, copies))
+    return bytes(head) + struct.pack("<q", table_at) + body + table
+
+
 ROOT = 1024.0
 
 

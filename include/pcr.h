@@ -354,6 +354,13 @@ typedef struct pcr_laz_info {
 
 /* Header, LASzip VLR and chunk table of a LAZ file (host only). */
 int pcr_laz_info_get(const void* bytes, uint64_t size, pcr_laz_info* out);
+/* The chunk table: chunk c holds records [chunk_first[c], chunk_first[c+1])
+ * in bytes [chunk_start[c], chunk_start[c+1]) of the file (num_chunks + 1
+ * entries; num_chunks = 0 when the file has no table; with capacity 0 and
+ * null arrays only the count is returned). For loaders that
+ * schedule their own chunk ranges, as LASreadPoint::seek finds chunks. */
+int pcr_laz_chunk_table(const void* bytes, uint64_t size, uint64_t* chunk_start, uint64_t* chunk_first,
+                        uint32_t capacity, uint32_t* num_chunks);
 /* Decode records [first_point, first_point + num_points) of a LAZ file into
  * `records` (num_points * record_length bytes, the uncompressed LAS records)
  * on `threads` host threads (<= 0: up to 16), one chunk per thread at a time.

@@ -211,3 +211,39 @@ def test_reference_laszip_decodes_the_fixtures_alike(records, name, tmp_path):
     ref = np.fromfile(out, np.uint8).reshape(rec.shape)
     assert np.array_equal(ref, rec)
     assert np.array_equal(pcr.laz_decode(laz(name)), ref)
+
+
+@pytest.mark.parametrize("name,copies", [("f0_chunk1000", 3), ("f3_extra3_variable", 5), ("f7_v4", 7),
+                                         ("f8_extra2_v3", 2)])
+def test_tiled_file_decodes_to_repeated_records(records, name, copies):
+    """datasets.laz_tile repeats a file's chunks behind a new arithmetic-coded
+    chunk table (variable chunk sizes)."""
+    from compute_rasterizer_amd.datasets import laz_tile
+    rec, _ = expected(records, name)
+    big = laz_tile(laz(name), copies)
+    info = pcr.laz_info(big)
+    assert info.chunk_size == 0xFFFFFFFF and info.num_points == copies * rec.shape[0]
+    starts, firsts = pcr.laz_chunk_table(big)
+    assert len(starts) == info.num_chunks + 1 and firsts[0] == 0 and firsts[-1] == info.num_points
+    assert np.all(np.diff(starts.astype(np.int64)) > 0) and np.all(np.diff(firsts.astype(np.int64)) > 0)
+    assert np.array_equal(pcr.laz_decode(big, threads=4), np.tile(rec, (copies, 1)))
+
+
+def test_chunk_table_of_a_fixture():
+    data = laz("f2_rgb_chunk1500")
+    starts, firsts = pcr.laz_chunk_table(data)
+    assert list(firsts) == [0, 1500, 3000, 4100]
+    p0 = struct.unpack_from("<I", data, 96)[0]
+    table = struct.unpack_from("<q", data, p0)[0]
+    assert starts[0] == p0 + 8 and starts[-1] == table
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TOOL), reason="oracle/_ref/laz_ref_tool is built only where the "
+                    "reference's sources are (make -C oracle ref)")
+def test_reference_laszip_reads_a_tiled_file(records, tmp_path):
+    from compute_rasterizer_amd.datasets import laz_tile
+    rec, _ = expected(records, "f7_v4")
+    src, out = tmp_path / "tiled.laz", tmp_path / "out.bin"
+    src.write_bytes(laz_tile(laz("f7_v4"), 3))
+    subprocess.run([REF_TOOL, "decompress", str(src), str(out)], check=True)
+    assert np.array_equal(np.fromfile(out, np.uint8).reshape(-1, rec.shape[1]), np.tile(rec, (3, 1)))
