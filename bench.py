@@ -787,6 +787,11 @@ def main():
         roof["traffic"] = int(prof["traffic_bytes_per_launch"])
         roof["traffic_over_algorithmic"] = round(prof["traffic_bytes_per_launch"] /
                                                  prof["algorithmic_bytes_per_launch"], 3)
+        # issue-side bound beside the HBM fraction, from the same profiled launches
+        for k in ("valu_busy_frac", "tcc_hit_frac"):
+            if k in prof:
+                roof[k] = prof[k]
+        roof["counters_source"] = "profiles/raster_tiled_city1e9_1080p_traffic.json (rocprofv3, the 64 views)"
     out = {
         "metric": "Gpoints/s (1e9-pt adaptive-precision 1920x1080 frames)",
         "value": round(value, 3),
