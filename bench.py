@@ -401,7 +401,10 @@ def layout_sweep(ctx, n_points, batch_sizes, nviews, frames):
 
 def poisson_phase(srv, ids, n_clouds, rate, seconds, seed):
     """Poisson(rate) requests for `seconds`, each for a random cloud, view and
-    resolution; latencies from submit to image ready (the server's clock)."""
+    resolution; latencies from submit to image ready (the server's clock). The
+    client takes each image in place from the server's pinned result memory
+    and releases it, so a slow client copy cannot hold the memory that
+    admission needs."""
     import threading
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16))
@@ -435,7 +438,8 @@ def poisson_phase(srv, ids, n_clouds, rate, seconds, seed):
             raise RuntimeError(f"serving: request {i} was not submitted within 60 s")
         if errors:
             raise errors[0]
-        _, ms = srv.wait(tickets[i])
+        _, ms = srv.wait_pinned(tickets[i])   # the image in place, in the server's pinned memory
+        srv.release(tickets[i])
         lat.append(ms)
     wall = time.perf_counter() - start
     th.join()
