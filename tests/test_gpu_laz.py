@@ -88,3 +88,24 @@ def test_laz_cloud_renders_like_its_las_cloud(ctx, store):
         cloud.close()
     assert (out[0] != pcr.FB_CLEAR).sum() > 100
     assert np.array_equal(out[0], out[1])
+
+
+def test_multi_chunk_laz_files_on_device(ctx, store):
+    """Larger LAZ files (hundreds of chunks, decoded on several host threads in
+    pinned slabs) encode like the oracle's encoding of the same LAS records."""
+    from compute_rasterizer_amd.datasets import laz_tile
+    datas, lases = [], []
+    for name, key, copies in (("f7_v4", "f7_v3", 60), ("f3_extra3_variable", "f3_extra3_variable", 41)):
+        laz, _ = pair(store, name)
+        datas.append(laz_tile(laz, copies))
+        rec = store[key + "_records"]
+        hdr = bytearray(store[key + "_las_header"].tobytes())
+        n = copies * rec.shape[0]
+        hdr[107:111] = np.uint32(n).tobytes()
+        if len(hdr) >= 255:
+            hdr[247:255] = np.uint64(n).tobytes()
+        lases.append(bytes(hdr) + np.tile(rec, (copies, 1)).tobytes())
+    print("points", [int(pcr.laz_info(d).num_points) for d in datas], flush=True)
+    cloud, first = ctx.from_las(datas)
+    check_layout(cloud, oracle_files(lases, [True, True]))
+    cloud.close()
