@@ -7,11 +7,10 @@
 //
 // Construction (host):
 //   1. Morton keys (21 bits/axis over the bounding cube), stable LSD radix sort.
-//   2. Level l keeps, inside every level-l node (Morton prefix of 3l bits), the
-//      point of lowest seeded priority in each subsampling cell of side
-//      cube/(div * 2^l) (Morton prefix of 3(log2 div + l) bits) among the points
-//      not yet taken by a coarser level. A node whose remaining points fit in
-//      max_node_points keeps all of them and becomes a leaf.
+//   2. Level l: a node (Morton prefix of 3l bits) whose points not yet taken
+//      by a coarser level fit in max_node_points keeps all of them and becomes
+//      a leaf; a larger node keeps a seeded Poisson-disk sample of them with
+//      spacing cube/(div * 2^l) (see the Poisson-disk helpers below).
 //   3. Nodes are emitted level by level in Morton order (parents precede
 //      children); a node with more than max_node_points points is split into
 //      chunks that share the node's header fields. first_point is 4-aligned.
@@ -61,6 +60,82 @@ static uint32_t quant30(float x, float mn, double size) {
     return v < 1073741823u ? v : 1073741823u;
 }
 
+// ---- Poisson-disk subsampling (shared with the device builder) -----------
+// A node above max_node_points at level l keeps a maximal set of its
+// unassigned points with no two closer than the level's cell side
+// r = cube / (div * 2^l), chosen by seeded priority. Distances are exact in
+// the 21-bit Morton coordinates. The node is cut into half-cells of side r/2
+// (any two points of one are closer than r, so a half-cell keeps at most one
+// sample), and the half-cells are visited in 27 phases by (x, y, z) mod 3:
+// half-cells of one phase are at least 3 apart, so their points are more
+// than r apart and the order inside a phase does not matter (the phase
+// groups of L.-Y. Wei, "Parallel Poisson disk sampling", SIGGRAPH 2008).
+// A half-cell keeps its lowest-priority unassigned point that lies at least
+// r from every sample already kept in the 124 half-cells around it (within
+// the node), so the set is maximal: every other point is within r of a
+// sample.
+struct PoissonCell {
+    size_t a, b;        // sorted positions [a, b) of the half-cell's points
+    uint64_t prefix;    // Morton prefix of the half-cell (3t bits)
+};
+static inline uint32_t compact3(uint64_t x) {   // every third bit, from bit 0
+    x &= 0x1249249249249249ull;
+    x = (x | (x >> 2)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x >> 4)) & 0x100F00F00F00F00Full;
+    x = (x | (x >> 8)) & 0x1F0000FF0000FFull;
+    x = (x | (x >> 16)) & 0x1F00000000FFFFull;
+    x = (x | (x >> 32)) & 0x1FFFFFull;
+    return (uint32_t)x;
+}
+// cell coordinates (x, y, z) of a Morton prefix of 3t bits (key = x<<2 | y<<1 | z per bit)
+static inline void morton_cell(uint64_t prefix, int t, uint32_t c[3]) {
+    (void)t;
+    c[0] = compact3(prefix >> 2);
+    c[1] = compact3(prefix >> 1);
+    c[2] = compact3(prefix);
+}
+static inline uint64_t morton_of(const uint32_t c[3]) {
+    return (spread3_host(c[0]) << 2) | (spread3_host(c[1]) << 1) | spread3_host(c[2]);
+}
+// the point's 21-bit coordinates packed as x | y << 21 | z << 42
+static inline uint64_t morton_point(uint64_t key) {
+    return (uint64_t)compact3(key >> 2) | ((uint64_t)compact3(key >> 1) << 21) | ((uint64_t)compact3(key) << 42);
+}
+static inline int poisson_phase(const uint32_t c[3]) { return (int)(c[0] % 3 + 3 * (c[1] % 3) + 9 * (c[2] % 3)); }
+// Samples kept in the 124 half-cells around c (same node: the top l = t-g-1
+// bits of each coordinate agree), through lookup(prefix, &value).
+template <class F>
+static int poisson_neighbours(const uint32_t c[3], int t, int gbits, F lookup, uint64_t* out) {
+    const int64_t lim = (int64_t)1 << t;
+    const int sh = gbits + 1;   // half-cell bits below the node
+    int n = 0;
+    for (int o = 0; o < 125; o++) {
+        if (o == 62) continue;
+        const int64_t d[3] = {o % 5 - 2, (o / 5) % 5 - 2, o / 25 - 2};
+        uint32_t nc[3];
+        bool ok = true;
+        for (int k = 0; k < 3; k++) {
+            const int64_t v = (int64_t)c[k] + d[k];
+            ok = ok && v >= 0 && v < lim && (v >> sh) == ((int64_t)c[k] >> sh);
+            nc[k] = (uint32_t)v;
+        }
+        uint64_t v;
+        if (ok && lookup(morton_of(nc), &v)) out[n++] = v;
+    }
+    return n;
+}
+static inline bool poisson_conflict(uint64_t p, const uint64_t* nb, int nn, int64_t r) {
+    for (int k = 0; k < nn; k++) {
+        int64_t d2 = 0;
+        for (int a = 0; a < 3; a++) {
+            const int64_t d = (int64_t)((p >> (21 * a)) & 0x1FFFFF) - (int64_t)((nb[k] >> (21 * a)) & 0x1FFFFF);
+            d2 += d * d;
+        }
+        if (d2 < r * r) return true;
+    }
+    return false;
+}
+
 struct OctreeLayout {
     std::vector<pcr_batch> hdr;
     std::vector<uint32_t> lo, med, hi, col;
@@ -102,14 +177,15 @@ static int build_octree_layout(const float* xyz, const uint32_t* rgba, uint64_t 
     }
     radix_sort_pairs(keys, idx);
 
-    // 2. level assignment
+    // 2. level assignment: leaves, then Poisson-disk samples of the larger nodes
     const uint64_t pkey = stream_key(o->seed, 5);
     std::vector<int8_t> lev(n, -1);
     uint64_t remaining = n;
     int deepest = 0;
+    std::unordered_map<uint64_t, uint64_t> accepted;   // half-cell prefix -> packed sample coordinates
     for (int l = 0; l <= max_depth && remaining; l++) {
         const int node_shift = 63 - 3 * l;
-        const int cell_shift = 63 - 3 * (gbits + l);
+        bool sampling = false;
         size_t a = 0;
         while (a < n) {
             const uint64_t node = l == 0 ? 0 : keys[a] >> node_shift;
@@ -122,24 +198,51 @@ static int build_octree_layout(const float* xyz, const uint32_t* rgba, uint64_t 
                     for (size_t i = a; i < b; i++) if (lev[i] < 0) lev[i] = (int8_t)l;
                     remaining -= un;
                 } else {
-                    size_t c = a;
-                    while (c < b) {
-                        const uint64_t cell = keys[c] >> cell_shift;
-                        size_t best = SIZE_MAX;
-                        uint64_t bp = ~0ull;
-                        size_t e = c;
-                        for (; e < b && (keys[e] >> cell_shift) == cell; e++) {
-                            if (lev[e] >= 0) continue;
-                            const uint64_t p = rnd(pkey, idx[e]);
-                            if (p < bp) { bp = p; best = e; }
-                        }
-                        if (best != SIZE_MAX) { lev[best] = (int8_t)l; remaining--; }
-                        c = e;
-                    }
+                    sampling = true;
                 }
             }
             a = b;
         }
+        if (!sampling) continue;
+        // the unassigned points left are those of nodes above max_node_points
+        const int t = gbits + l + 1;                        // half-cell: 2^-t of the cube per axis
+        const int64_t r = (int64_t)1 << (21 - gbits - l);   // sample spacing in 2^-21 cube units
+        std::vector<PoissonCell> cells;
+        for (size_t c = 0; c < n;) {
+            const uint64_t pre = keys[c] >> (63 - 3 * t);
+            size_t e = c;
+            bool any = false;
+            for (; e < n && (keys[e] >> (63 - 3 * t)) == pre; e++) any |= lev[e] < 0;
+            if (any) cells.push_back(PoissonCell{c, e, pre});
+            c = e;
+        }
+        accepted.clear();
+        for (int phase = 0; phase < 27; phase++)
+            for (const PoissonCell& cell : cells) {
+                uint32_t cc[3];
+                morton_cell(cell.prefix, t, cc);
+                if (poisson_phase(cc) != phase) continue;
+                uint64_t nb[124];
+                const int nn = poisson_neighbours(cc, t, gbits, [&](uint64_t key, uint64_t* v) {
+                    auto it = accepted.find(key);
+                    if (it == accepted.end()) return false;
+                    *v = it->second;
+                    return true;
+                }, nb);
+                size_t best = SIZE_MAX;
+                uint64_t bp = ~0ull;
+                for (size_t i = cell.a; i < cell.b; i++) {
+                    if (lev[i] >= 0) continue;
+                    const uint64_t pq = morton_point(keys[i]);
+                    if (poisson_conflict(pq, nb, nn, r)) continue;
+                    const uint64_t pr = rnd(pkey, idx[i]);
+                    if (pr < bp) { bp = pr; best = i; }   // first minimum: the lower position wins ties
+                }
+                if (best == SIZE_MAX) continue;
+                lev[best] = (int8_t)l;
+                remaining--;
+                accepted[cell.prefix] = morton_point(keys[best]);
+            }
     }
 
     // 3. nodes, level by level in Morton order

@@ -2,12 +2,12 @@
 //
 // Same layout, same bytes as the host builder (a GPU test compares them):
 //   1. Morton keys over the bounding cube + hipCUB radix sort (stable).
-//   2. per level l: node and subsampling-cell segments of the sorted keys are
+//   2. per level l: node and half-cell segments of the sorted keys are
 //      numbered by prefix sums of "starts a new segment" flags; a node whose
 //      unassigned points fit in max_node_points (or the last level) keeps them
-//      all; otherwise each cell keeps its lowest-priority unassigned point
-//      (64-bit atomicMin of the priority, then of the sorted position, which
-//      is the host builder's first-minimum tie rule).
+//      all; the larger nodes keep a Poisson-disk sample: 27 phase launches, one
+//      wave per half-cell of the phase, the samples kept so far in a hash table
+//      keyed by half-cell (the host builder's rule and tie order).
 //   3. a stable radix sort by level groups the points level by level in
 //      Morton order; node and chunk segments are numbered again, the chunk
 //      table goes to the host, which computes parents, spacing, 4-aligned
@@ -103,32 +103,155 @@ __global__ void k_oct_leaf(const uint32_t* __restrict__ nid, const uint32_t* __r
     wave_add(assigned, mine);
 }
 
-// lowest priority per subsampling cell; the plain read skips most atomics
-// (it can only be stale-high: the value only decreases)
-__global__ void k_oct_cellmin(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ cid,
-                              const int8_t* __restrict__ lev, uint64_t n, uint64_t pkey,
-                              unsigned long long* __restrict__ cellmin) {
+// ---- Poisson-disk subsampling (the host builder's rule, pcr_octree.hip) ---
+__device__ __forceinline__ uint32_t compact3_d(uint64_t x) {
+    x &= 0x1249249249249249ull;
+    x = (x | (x >> 2)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x >> 4)) & 0x100F00F00F00F00Full;
+    x = (x | (x >> 8)) & 0x1F0000FF0000FFull;
+    x = (x | (x >> 16)) & 0x1F00000000FFFFull;
+    x = (x | (x >> 32)) & 0x1FFFFFull;
+    return (uint32_t)x;
+}
+__device__ __forceinline__ uint64_t point21(uint64_t key) {
+    return (uint64_t)compact3_d(key >> 2) | ((uint64_t)compact3_d(key >> 1) << 21) | ((uint64_t)compact3_d(key) << 42);
+}
+__device__ __forceinline__ uint32_t hslot(uint64_t key, int cap_bits) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - cap_bits));
+}
+// half-cell starts over the sorted points, and the half-cells that still
+// hold an unassigned point (nonzero = a candidate cell)
+__global__ void k_ps_cells(const uint32_t* __restrict__ cid, const int8_t* __restrict__ lev, uint64_t n,
+                           uint32_t* __restrict__ cstart, uint32_t* __restrict__ cand) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        if (lev[i] >= 0) continue;
-        const unsigned long long p = rnd(pkey, perm[i]);
-        unsigned long long* c = cellmin + cid[i] - 1;
-        if (p < *c) atomicMin(c, p);
+        const uint32_t c = cid[i] - 1;
+        if (i == 0 || cid[i - 1] != cid[i]) cstart[c] = (uint32_t)i;
+        if (lev[i] < 0) cand[c] = 1u;   // benign race: every writer stores 1
     }
 }
-
-__global__ void k_oct_cellwin(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ cid,
-                              const int8_t* __restrict__ lev, uint64_t n, uint64_t pkey,
-                              const unsigned long long* __restrict__ cellmin, uint32_t* __restrict__ cellwin) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        if (lev[i] < 0 && rnd(pkey, perm[i]) == cellmin[cid[i] - 1]) atomicMin(cellwin + cid[i] - 1, (uint32_t)i);
+__device__ __forceinline__ int cell_phase(const uint64_t* __restrict__ K, const uint32_t* __restrict__ cstart,
+                                          uint64_t c, int t) {
+    const uint64_t pre = K[cstart[c]] >> (63 - 3 * t);
+    return (int)(compact3_d(pre >> 2) % 3 + 3 * (compact3_d(pre >> 1) % 3) + 9 * (compact3_d(pre) % 3));
 }
-
-__global__ void k_oct_assign(const uint32_t* __restrict__ cid, int8_t* __restrict__ lev, uint64_t n, int l,
-                             const uint32_t* __restrict__ cellwin, unsigned long long* __restrict__ assigned) {
+// candidate half-cells per phase (LDS histogram, one global add per phase per block)
+__global__ __launch_bounds__(256) void k_ps_hist(const uint64_t* __restrict__ K, const uint32_t* __restrict__ cstart,
+                                                 const uint32_t* __restrict__ cand, uint64_t ncells, int t,
+                                                 uint32_t* __restrict__ counts) {
+    __shared__ uint32_t h[27];
+    if (threadIdx.x < 27) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += (uint64_t)gridDim.x * blockDim.x)
+        if (cand[c]) atomicAdd(&h[cell_phase(K, cstart, c, t)], 1u);
+    __syncthreads();
+    if (threadIdx.x < 27 && h[threadIdx.x]) atomicAdd(counts + threadIdx.x, h[threadIdx.x]);
+}
+// the candidate half-cells listed by phase: cursor[p] starts at phase p's offset
+__global__ __launch_bounds__(256) void k_ps_scatter(const uint64_t* __restrict__ K, const uint32_t* __restrict__ cstart,
+                                                    const uint32_t* __restrict__ cand, uint64_t ncells, int t,
+                                                    uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
+    __shared__ uint32_t h[27], base[27];
+    for (uint64_t c0 = (uint64_t)blockIdx.x * blockDim.x; c0 < ncells; c0 += (uint64_t)gridDim.x * blockDim.x) {
+        if (threadIdx.x < 27) h[threadIdx.x] = 0;
+        __syncthreads();
+        const uint64_t c = c0 + threadIdx.x;
+        int ph = -1;
+        uint32_t slot = 0;
+        if (c < ncells && cand[c]) {
+            ph = cell_phase(K, cstart, c, t);
+            slot = atomicAdd(&h[ph], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 27 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(cursor + threadIdx.x, h[threadIdx.x]);
+        __syncthreads();
+        if (ph >= 0) list[base[ph] + slot] = (uint32_t)c;
+        __syncthreads();
+    }
+}
+// One wave per half-cell of the phase: gather the samples kept in its 124
+// neighbours (same node) into LDS, then every lane tests its share of the
+// cell's unassigned points against them; the lowest (priority, position)
+// among the free ones becomes the cell's sample.
+__global__ __launch_bounds__(256) void k_ps_phase(const uint64_t* __restrict__ K, const uint32_t* __restrict__ perm,
+                                                  int8_t* __restrict__ lev, const uint32_t* __restrict__ cstart,
+                                                  const uint32_t* __restrict__ list, uint32_t nlist,
+                                                  uint64_t ncells, uint64_t n, int l, int t, int gbits, uint64_t pkey,
+                                                  unsigned long long* __restrict__ hkey,
+                                                  unsigned long long* __restrict__ hval, int cap_bits,
+                                                  unsigned long long* __restrict__ assigned) {
+    __shared__ unsigned long long s_nb[4][124];
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)1 << (21 - gbits - l);
+    const int64_t lim = (int64_t)1 << t;
+    const int sh = gbits + 1;
+    const uint32_t capmask = (1u << cap_bits) - 1u;
     uint32_t mine = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        if (lev[i] < 0 && cellwin[cid[i] - 1] == (uint32_t)i) { lev[i] = (int8_t)l; mine++; }
-    wave_add(assigned, mine);
+    for (uint64_t e = (uint64_t)blockIdx.x * 4 + w; e < nlist; e += (uint64_t)gridDim.x * 4) {
+        const uint64_t c = list[e];                               // wave-uniform
+        const uint64_t a = cstart[c], b = c + 1 < ncells ? cstart[c + 1] : n;
+        const uint64_t pre = K[a] >> (63 - 3 * t);
+        const uint32_t cc[3] = {compact3_d(pre >> 2), compact3_d(pre >> 1), compact3_d(pre)};
+        for (int j = (int)lane; j < 124; j += 64) {
+            const int o = j < 62 ? j : j + 1;
+            const int64_t d[3] = {o % 5 - 2, (o / 5) % 5 - 2, o / 25 - 2};
+            bool ok = true;
+            uint64_t nk = 0;
+            for (int k = 0; k < 3; k++) {
+                const int64_t v = (int64_t)cc[k] + d[k];
+                ok = ok && v >= 0 && v < lim && (v >> sh) == ((int64_t)cc[k] >> sh);
+                nk |= spread3_d((uint32_t)v) << (2 - k);
+            }
+            unsigned long long val = ~0ull;
+            if (ok) {
+                const unsigned long long key = nk + 1;
+                for (uint32_t sl = hslot(key, cap_bits);; sl = (sl + 1) & capmask) {
+                    const unsigned long long hk = hkey[sl];
+                    if (hk == key) { val = hval[sl]; break; }
+                    if (hk == 0) break;
+                }
+            }
+            s_nb[w][j] = val;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        unsigned long long bp = ~0ull;
+        uint32_t bi = 0xFFFFFFFFu;
+        for (uint64_t i = a + lane; i < b; i += 64) {
+            if (lev[i] >= 0) continue;
+            const uint64_t p = point21(K[i]);
+            bool free = true;
+            for (int j = 0; j < 124 && free; j++) {
+                const unsigned long long v = s_nb[w][j];
+                if (v == ~0ull) continue;
+                int64_t d2 = 0;
+                for (int k = 0; k < 3; k++) {
+                    const int64_t d = (int64_t)((p >> (21 * k)) & 0x1FFFFF) - (int64_t)((v >> (21 * k)) & 0x1FFFFF);
+                    d2 += d * d;
+                }
+                free = d2 >= r * r;
+            }
+            if (!free) continue;
+            const unsigned long long pr = rnd(pkey, perm[i]);
+            if (pr < bp || (pr == bp && (uint32_t)i < bi)) { bp = pr; bi = (uint32_t)i; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long op = __shfl_xor(bp, off);
+            const uint32_t oi = __shfl_xor(bi, off);
+            if (op < bp || (op == bp && oi < bi)) { bp = op; bi = oi; }
+        }
+        if (lane == 0 && bi != 0xFFFFFFFFu) {
+            lev[bi] = (int8_t)l;
+            mine++;
+            const unsigned long long key = pre + 1;
+            for (uint32_t sl = hslot(key, cap_bits);; sl = (sl + 1) & capmask) {
+                const unsigned long long prev = atomicCAS(hkey + sl, 0ull, key);
+                if (prev == 0ull || prev == key) { hval[sl] = point21(K[bi]); break; }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (lane == 0 && mine) atomicAdd(assigned, (unsigned long long)mine);
 }
 
 __global__ void k_oct_iota(uint32_t* __restrict__ v, uint64_t n) {
@@ -304,7 +427,7 @@ int build_octree_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, ui
     // 2. level assignment
     int8_t* lev = nullptr;
     uint32_t *nflag = nullptr, *cflag = nullptr, *nid = nullptr, *cid = nullptr, *ncount = nullptr, *cellwin = nullptr;
-    unsigned long long *cellmin = nullptr, *assigned = nullptr;
+    unsigned long long* assigned = nullptr;
     PCR_HIP(B.alloc(&lev, n));
     PCR_HIP(B.alloc(&nflag, n));
     PCR_HIP(B.alloc(&cflag, n));
@@ -312,8 +435,14 @@ int build_octree_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, ui
     PCR_HIP(B.alloc(&cid, n));
     PCR_HIP(B.alloc(&ncount, n));
     PCR_HIP(B.alloc(&cellwin, n));
-    PCR_HIP(B.alloc(&cellmin, n));
     PCR_HIP(B.alloc(&assigned, 1));
+    uint32_t *cellcand = nullptr, *plist = nullptr, *phase_cnt = nullptr, *phase_cur = nullptr;
+    unsigned long long *hkey = nullptr, *hval = nullptr;
+    uint64_t hcap = 0;
+    PCR_HIP(B.alloc(&cellcand, n));
+    PCR_HIP(B.alloc(&plist, n));
+    PCR_HIP(B.alloc(&phase_cnt, 27));
+    PCR_HIP(B.alloc(&phase_cur, 27));
     PCR_HIP(hipMemsetAsync(lev, 0xFF, n, s));
     PCR_HIP(hipMemsetAsync(assigned, 0, 8, s));
     size_t tscan = 0;
@@ -323,7 +452,8 @@ int build_octree_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, ui
     const uint64_t pkey = stream_key(o->seed, 5);
     int deepest = 0;
     for (int l = 0; l <= max_depth; l++) {
-        k_oct_flags<<<G, 256, 0, s>>>(K, n, l, gbits, nflag, cflag);
+        // cells: the half-cells of level l (at a last level past 20 - gbits only the node flags matter)
+        k_oct_flags<<<G, 256, 0, s>>>(K, n, l, std::min(gbits + 1, 21 - l), nflag, cflag);
         PCR_HIP(hipcub::DeviceScan::InclusiveSum(stemp, tscan, nflag, nid, (int)n, s));
         PCR_HIP(hipcub::DeviceScan::InclusiveSum(stemp, tscan, cflag, cid, (int)n, s));
         // per-node unassigned counts as prefix-sum differences: cflag is free
@@ -332,16 +462,49 @@ int build_octree_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, ui
         PCR_HIP(hipcub::DeviceScan::InclusiveSum(stemp, tscan, cflag, ncount, (int)n, s));
         k_oct_nstart<<<G, 256, 0, s>>>(nflag, nid, n, cellwin);
         k_oct_leaf<<<G, 256, 0, s>>>(nid, ncount, cellwin, lev, n, maxpts, l, l == max_depth, assigned);
-        k_oct_fill64<<<G, 256, 0, s>>>(cellmin, n, ~0ull);
-        k_oct_fill32<<<G, 256, 0, s>>>(cellwin, n, ~0u);
-        k_oct_cellmin<<<G, 256, 0, s>>>(perm, cid, lev, n, pkey, cellmin);
-        k_oct_cellwin<<<G, 256, 0, s>>>(perm, cid, lev, n, pkey, cellmin, cellwin);
-        k_oct_assign<<<G, 256, 0, s>>>(cid, lev, n, l, cellwin, assigned);
         PCR_HIP(hipGetLastError());
         unsigned long long done = 0;
         PCR_HIP(hipMemcpyAsync(&done, assigned, 8, hipMemcpyDeviceToHost, s));
         PCR_HIP(hipStreamSynchronize(s));
         deepest = l;
+        if (done >= n) break;
+        // Poisson-disk samples of the nodes above maxpts (the rest of the points)
+        const int t = gbits + l + 1;
+        uint32_t ncells = 0;
+        PCR_HIP(hipMemcpyAsync(&ncells, cid + n - 1, 4, hipMemcpyDeviceToHost, s));
+        k_oct_fill32<<<G, 256, 0, s>>>(cellcand, n, 0u);
+        PCR_HIP(hipStreamSynchronize(s));
+        k_ps_cells<<<G, 256, 0, s>>>(cid, lev, n, cellwin, cellcand);
+        k_oct_fill32<<<1, 64, 0, s>>>(phase_cnt, 27, 0u);
+        k_ps_hist<<<G, 256, 0, s>>>(K, cellwin, cellcand, ncells, t, phase_cnt);
+        uint32_t cnt[27], off[28];
+        PCR_HIP(hipMemcpyAsync(cnt, phase_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        PCR_HIP(hipStreamSynchronize(s));
+        off[0] = 0;
+        for (int ph = 0; ph < 27; ph++) off[ph + 1] = off[ph] + cnt[ph];
+        const uint64_t nc = off[27];
+        PCR_HIP(hipMemcpyAsync(phase_cur, off, 27 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        k_ps_scatter<<<G, 256, 0, s>>>(K, cellwin, cellcand, ncells, t, phase_cur, plist);
+        int cap_bits = 6;
+        while ((1ull << cap_bits) < 2 * nc) cap_bits++;
+        PCR_CHECK(cap_bits <= 31, "octree_device: %llu sample cells exceed the hash table", nc);
+        const uint64_t cap = 1ull << cap_bits;
+        if (cap > hcap) {
+            PCR_HIP(B.alloc(&hkey, cap));
+            PCR_HIP(B.alloc(&hval, cap));
+            hcap = cap;
+        }
+        k_oct_fill64<<<G, 256, 0, s>>>(hkey, cap, 0ull);
+        for (int phase = 0; phase < 27; phase++) {
+            if (!cnt[phase]) continue;
+            const uint64_t gw = (cnt[phase] + 3) / 4;
+            const int GP = (int)std::min<uint64_t>(gw, (uint64_t)ctx->cu_count * 32);
+            k_ps_phase<<<GP, 256, 0, s>>>(K, perm, lev, cellwin, plist + off[phase], cnt[phase], ncells, n, l, t,
+                                          gbits, pkey, hkey, hval, cap_bits, assigned);
+        }
+        PCR_HIP(hipGetLastError());
+        PCR_HIP(hipMemcpyAsync(&done, assigned, 8, hipMemcpyDeviceToHost, s));
+        PCR_HIP(hipStreamSynchronize(s));
         if (done >= n) break;
     }
 

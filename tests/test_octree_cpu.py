@@ -54,24 +54,69 @@ def test_hierarchy_parents_levels_spacing(tree):
     assert nodes["level"].max() >= 2
 
 
-def test_one_point_per_subsampling_cell_in_inner_nodes(tree):
+def _nodes_of(nodes):
+    """Header indices grouped into nodes (a node's chunks are consecutive and
+    share level, parent and box); returns (groups, index of each header's group)."""
+    groups, of = [], np.zeros(len(nodes), np.int64)
+    for k, nd in enumerate(nodes):
+        if groups:
+            h = nodes[groups[-1][0]]
+            if (h["level"] == nd["level"] and h["parent"] == nd["parent"] and np.array_equal(h["min"], nd["min"])
+                    and np.array_equal(h["max"], nd["max"])):
+                groups[-1].append(k)
+                of[k] = len(groups) - 1
+                continue
+        groups.append([k])
+        of[k] = len(groups) - 1
+    return groups, of
+
+
+def test_inner_nodes_hold_maximal_poisson_disk_samples(tree):
+    """A node with children holds a seeded Poisson-disk sample of the points of
+    its region: no two of its points closer than r = cube / (128 * 2^level)
+    (exactly, in the 21-bit Morton coordinates), and every point stored deeper
+    in its subtree within r of one of them (maximal)."""
+    from scipy.spatial import cKDTree
     xyz, ids, (nodes, lo, med, hi, col) = tree
     mn = xyz.min(0)
     side = float(max(np.float32(xyz.max(0)[a] - mn[a]) for a in range(3)))
     scale = 2097152.0 / side
     q = np.minimum(((xyz.astype(np.float64) - mn.astype(np.float64)) * scale).astype(np.int64), 2097151)
-    has_child = set(int(p) for p in nodes["parent"] if p >= 0)
-    checked = 0
+    groups, of = _nodes_of(nodes)
+    children = [[] for _ in groups]
     for k, nd in enumerate(nodes):
-        if k not in has_child:
+        if nd["parent"] >= 0 and k == groups[of[k]][0]:
+            children[of[nd["parent"]]].append(of[k])
+
+    def points(g):
+        return np.concatenate([col[int(nodes[k]["first_point"]):int(nodes[k]["first_point"]) +
+                                   int(nodes[k]["num_points"])] for k in groups[g]])
+
+    def subtree(g):
+        out, stack = [], list(children[g])
+        while stack:
+            c = stack.pop()
+            out.append(points(c))
+            stack += children[c]
+        return np.concatenate(out)
+
+    checked = 0
+    for g in range(len(groups)):
+        if not children[g]:
             continue
-        lv = int(nd["level"])
-        f, n = int(nd["first_point"]), int(nd["num_points"])
-        cells = q[col[f:f + n]] >> (21 - 7 - lv)
-        keyset = {tuple(c) for c in cells}
-        assert len(keyset) == n, f"node {k}: two points share a level-{lv} subsampling cell"
+        lv = int(nodes[groups[g][0]]["level"])
+        r = 1 << (21 - 7 - lv)
+        s = q[points(g)]
+        t = cKDTree(s)
+        for i, j in t.query_pairs(r + 1.0):
+            d2 = int(((s[i] - s[j]) ** 2).sum())
+            assert d2 >= r * r, f"node {g}: two samples {d2 ** 0.5:.1f} apart, spacing {r}"
+        deeper = q[subtree(g)]
+        dist, nearest = t.query(deeper)
+        d2 = ((deeper - s[nearest]) ** 2).sum(1)
+        assert np.all(d2 < r * r), f"node {g}: a point of its subtree is {np.sqrt(d2.max()):.1f} from every sample"
         checked += 1
-    assert checked >= 1
+    assert checked >= 3
 
 
 def test_layout_is_deterministic_and_seed_dependent():
