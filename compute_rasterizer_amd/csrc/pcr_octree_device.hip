@@ -175,9 +175,8 @@ __global__ __launch_bounds__(256) void k_ps_scatter(const uint64_t* __restrict__
 __global__ __launch_bounds__(256) void k_ps_phase(const uint64_t* __restrict__ K, const uint32_t* __restrict__ perm,
                                                   int8_t* __restrict__ lev, const uint32_t* __restrict__ cstart,
                                                   const uint32_t* __restrict__ list, uint32_t nlist,
-                                                  uint64_t ncells, uint64_t n, int l, int t, int gbits, uint64_t pkey,
-                                                  unsigned long long* __restrict__ hkey,
-                                                  unsigned long long* __restrict__ hval, int cap_bits,
+                                                  uint64_t ncells, uint64_t n, int l, int t, int gbits, int phase,
+                                                  uint64_t pkey, unsigned long long* __restrict__ htab, int cap_bits,
                                                   unsigned long long* __restrict__ assigned) {
     __shared__ unsigned long long s_nb[4][124];
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -191,26 +190,39 @@ __global__ __launch_bounds__(256) void k_ps_phase(const uint64_t* __restrict__ K
         const uint64_t a = cstart[c], b = c + 1 < ncells ? cstart[c + 1] : n;
         const uint64_t pre = K[a] >> (63 - 3 * t);
         const uint32_t cc[3] = {compact3_d(pre >> 2), compact3_d(pre >> 1), compact3_d(pre)};
-        for (int j = (int)lane; j < 124; j += 64) {
+        // the kept samples around the cell, compacted to s_nb[w][0 .. m)
+        unsigned long long found[2] = {~0ull, ~0ull};
+        for (int j = (int)lane, q = 0; j < 124; j += 64, q++) {
             const int o = j < 62 ? j : j + 1;
             const int64_t d[3] = {o % 5 - 2, (o / 5) % 5 - 2, o / 25 - 2};
             bool ok = true;
             uint64_t nk = 0;
+            int nph = 0, mul = 1;
             for (int k = 0; k < 3; k++) {
                 const int64_t v = (int64_t)cc[k] + d[k];
                 ok = ok && v >= 0 && v < lim && (v >> sh) == ((int64_t)cc[k] >> sh);
                 nk |= spread3_d((uint32_t)v) << (2 - k);
+                nph += mul * (int)(((v % 3) + 3) % 3);
+                mul *= 3;
             }
+            // only half-cells of earlier phases can hold a sample yet
             unsigned long long val = ~0ull;
-            if (ok) {
+            if (ok && nph < phase) {
                 const unsigned long long key = nk + 1;
                 for (uint32_t sl = hslot(key, cap_bits);; sl = (sl + 1) & capmask) {
-                    const unsigned long long hk = hkey[sl];
-                    if (hk == key) { val = hval[sl]; break; }
+                    const unsigned long long hk = htab[2 * (uint64_t)sl];
+                    if (hk == key) { val = htab[2 * (uint64_t)sl + 1]; break; }
                     if (hk == 0) break;
                 }
             }
-            s_nb[w][j] = val;
+            found[q] = val;
+        }
+        int m = 0;
+        for (int q = 0; q < 2; q++) {
+            const unsigned long long has = __ballot(found[q] != ~0ull);
+            const int before = __popcll(has & ((1ull << lane) - 1ull));
+            if (found[q] != ~0ull) s_nb[w][m + before] = found[q];
+            m += __popcll(has);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -220,9 +232,8 @@ __global__ __launch_bounds__(256) void k_ps_phase(const uint64_t* __restrict__ K
             if (lev[i] >= 0) continue;
             const uint64_t p = point21(K[i]);
             bool free = true;
-            for (int j = 0; j < 124 && free; j++) {
+            for (int j = 0; j < m && free; j++) {
                 const unsigned long long v = s_nb[w][j];
-                if (v == ~0ull) continue;
                 int64_t d2 = 0;
                 for (int k = 0; k < 3; k++) {
                     const int64_t d = (int64_t)((p >> (21 * k)) & 0x1FFFFF) - (int64_t)((v >> (21 * k)) & 0x1FFFFF);
@@ -244,8 +255,8 @@ __global__ __launch_bounds__(256) void k_ps_phase(const uint64_t* __restrict__ K
             mine++;
             const unsigned long long key = pre + 1;
             for (uint32_t sl = hslot(key, cap_bits);; sl = (sl + 1) & capmask) {
-                const unsigned long long prev = atomicCAS(hkey + sl, 0ull, key);
-                if (prev == 0ull || prev == key) { hval[sl] = point21(K[bi]); break; }
+                const unsigned long long prev = atomicCAS(htab + 2 * (uint64_t)sl, 0ull, key);
+                if (prev == 0ull || prev == key) { htab[2 * (uint64_t)sl + 1] = point21(K[bi]); break; }
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -437,7 +448,7 @@ int build_octree_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, ui
     PCR_HIP(B.alloc(&cellwin, n));
     PCR_HIP(B.alloc(&assigned, 1));
     uint32_t *cellcand = nullptr, *plist = nullptr, *phase_cnt = nullptr, *phase_cur = nullptr;
-    unsigned long long *hkey = nullptr, *hval = nullptr;
+    unsigned long long* htab = nullptr;   // (half-cell key + 1, packed sample) pairs
     uint64_t hcap = 0;
     PCR_HIP(B.alloc(&cellcand, n));
     PCR_HIP(B.alloc(&plist, n));
@@ -490,17 +501,16 @@ int build_octree_device(pcr_ctx* ctx, const float* xyz, const uint32_t* rgba, ui
         PCR_CHECK(cap_bits <= 31, "octree_device: %llu sample cells exceed the hash table", nc);
         const uint64_t cap = 1ull << cap_bits;
         if (cap > hcap) {
-            PCR_HIP(B.alloc(&hkey, cap));
-            PCR_HIP(B.alloc(&hval, cap));
+            PCR_HIP(B.alloc(&htab, 2 * cap));
             hcap = cap;
         }
-        k_oct_fill64<<<G, 256, 0, s>>>(hkey, cap, 0ull);
+        k_oct_fill64<<<G, 256, 0, s>>>(htab, 2 * cap, 0ull);
         for (int phase = 0; phase < 27; phase++) {
             if (!cnt[phase]) continue;
             const uint64_t gw = (cnt[phase] + 3) / 4;
             const int GP = (int)std::min<uint64_t>(gw, (uint64_t)ctx->cu_count * 32);
             k_ps_phase<<<GP, 256, 0, s>>>(K, perm, lev, cellwin, plist + off[phase], cnt[phase], ncells, n, l, t,
-                                          gbits, pkey, hkey, hval, cap_bits, assigned);
+                                          gbits, phase, pkey, htab, cap_bits, assigned);
         }
         PCR_HIP(hipGetLastError());
         PCR_HIP(hipMemcpyAsync(&done, assigned, 8, hipMemcpyDeviceToHost, s));
