@@ -55,7 +55,7 @@ __global__ void k_oct_keys(const float* __restrict__ xyz, uint64_t n, float mn0,
 
 __device__ __forceinline__ uint64_t prefix(uint64_t k, int bits3) { return bits3 == 0 ? 0ull : k >> (63 - bits3); }
 
-// flags of "first point of a node / of a subsampling cell" at level l
+// flags of "first point of a node / of a half-cell" at level l (cell bits g: 3(g + l) Morton bits)
 __global__ void k_oct_flags(const uint64_t* __restrict__ K, uint64_t n, int l, int g, uint32_t* __restrict__ nflag,
                             uint32_t* __restrict__ cflag) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {

@@ -422,11 +422,15 @@ typedef struct pcr_octree_opts {
 
 /* Build a Potree-style octree from points on the host (PotreeData.h:188-317
  * stands in as the node layout). Nodes become pcr_batch headers with
- * level/parent/spacing set; node chunks are stored in node order. */
+ * level/parent/spacing set; node chunks are stored in node order. A node that
+ * holds more than max_node_points of the points no coarser level took keeps a
+ * seeded Poisson-disk sample of them (no two closer than cube side /
+ * (root_spacing_div * 2^level), maximal) and passes the rest down; a smaller
+ * node keeps them all. */
 int pcr_cloud_build_octree_host(pcr_ctx* ctx, const float* xyz_host, const uint32_t* rgba_host,
                                 uint64_t n, const pcr_octree_opts* opts, pcr_cloud** out);
 /* Same octree, byte for byte, built on the device from points already there
- * (Morton sort, per-level subsampling and encoding on the device; only the
+ * (Morton sort, per-level Poisson-disk sampling and encoding on the device; only the
  * per-chunk table visits the host). n < 2^31. */
 int pcr_octree_generate(pcr_ctx* ctx, const pcr_gen_cfg* cfg, const pcr_octree_opts* opts, pcr_cloud** out);
 int pcr_cloud_build_octree_device(pcr_ctx* ctx, const float* xyz_dev, const uint32_t* rgba_dev,
