@@ -534,9 +534,11 @@ __device__ __forceinline__ uint32_t tiled_batch(uint32_t bi, int level, const pc
         __syncthreads();
         tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, pj, fb, tile, x0, y0, tw, th, natom, interior);
         __syncthreads();
-        // Four touched pixels per thread at a time: their compare loads are
-        // all issued before the first is needed, so a thread waits once per
-        // four pixels (fb_min's compare and atomic, reordered across pixels).
+        // Four touched pixels per thread at a time (fb_min's compare and
+        // atomic, reordered across pixels). The compiler keeps each compare
+        // load in the branch of its pixel, with a wait after it; issuing the
+        // four unconditionally, so that they are in flight together, measured
+        // the same: the other waves of the SIMD already cover the wait.
         // row = i / tw as a multiply-high by ceil(2^32 / tw): exact because
         // i < 2^13 and 2 <= tw <= 2^13 keep the error below 1/tw
         const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
