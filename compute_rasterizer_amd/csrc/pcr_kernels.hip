@@ -772,11 +772,32 @@ struct TileBand {
     bool first;
 };
 
+// Pass 2's sums for one tile pixel, carried in registers over the four
+// points of a thread's quad. The quad's points are consecutive in the
+// batch's Morton order, and in a far batch most of them land on one pixel:
+// a run of them is added to the tile as one pair of ds_add_u64 instead of
+// one pair per point. The LDS performs adds on one word one after the other,
+// so a far batch's adds (hundreds of points per pixel) queued on few words.
+// At most 4 x 255 per channel is held in a register, so a half never carries
+// into the other, and the tile word gets the same sum.
+struct PendingSums {
+    uint32_t t = 0xFFFFFFFFu;          // tile pixel, or none
+    unsigned long long rg = 0, bc = 0; // R | G << 32, B | count << 32
+};
+__device__ __forceinline__ void pending_flush(PendingSums& p, unsigned long long* tacc) {
+    if (p.t != 0xFFFFFFFFu) {
+        lds_add64(tacc + 2 * p.t, p.rg);
+        lds_add64(tacc + 2 * p.t + 1, p.bc);
+        p.t = 0xFFFFFFFFu;
+    }
+}
+
 // A point in the band's rows of the rectangle: its tile pixel (pass 1) or,
-// if visible, its sums (pass 2).
+// if visible, its sums (pass 2, joined to the pending run of its thread).
 template <int PASS>
 __device__ __forceinline__ void hqs_band_point(uint32_t ux, uint32_t ry, uint32_t dep, uint32_t color,
-                                               uint32_t* tdepth, unsigned long long* tacc, const TileBand& tb) {
+                                               uint32_t* tdepth, unsigned long long* tacc, const TileBand& tb,
+                                               PendingSums& pend) {
     const uint32_t by = ry - (uint32_t)tb.boff;
     if (by >= (uint32_t)tb.bh) return;
     const uint32_t t = by * (uint32_t)tb.tw + ux;
@@ -784,8 +805,17 @@ __device__ __forceinline__ void hqs_band_point(uint32_t ux, uint32_t ry, uint32_
         lds_min32(tdepth + t, dep);
     } else {
         if (!(__uint_as_float(dep) <= __uint_as_float(tdepth[t]) * 1.01f)) return;
-        lds_add64(tacc + 2 * t, (unsigned long long)(color & 255u) | ((unsigned long long)((color >> 8) & 255u) << 32));
-        lds_add64(tacc + 2 * t + 1, (unsigned long long)((color >> 16) & 255u) | (1ull << 32));
+        const unsigned long long rg = (unsigned long long)(color & 255u) | ((unsigned long long)((color >> 8) & 255u) << 32);
+        const unsigned long long bc = (unsigned long long)((color >> 16) & 255u) | (1ull << 32);
+        if (t != pend.t) {
+            pending_flush(pend, tacc);
+            pend.t = t;
+            pend.rg = rg;
+            pend.bc = bc;
+        } else {
+            pend.rg += rg;
+            pend.bc += bc;
+        }
     }
 }
 
@@ -796,7 +826,8 @@ template <int PASS, int LC, bool TILE, bool INTERIOR = false>
 __device__ __forceinline__ void hqs_tile_point(const Decode& d, uint32_t lo, uint32_t med, uint32_t hi,
                                                uint32_t color, uint32_t local, uint32_t np, const Proj& pj,
                                                uint32_t* __restrict__ depth, uint32_t* __restrict__ acc,
-                                               uint32_t* tdepth, unsigned long long* tacc, const TileBand& tb) {
+                                               uint32_t* tdepth, unsigned long long* tacc, const TileBand& tb,
+                                               PendingSums& pend) {
     if (local >= np) return;
     float p[3];
     decode<LC>(d, lo, med, hi, p);
@@ -804,7 +835,7 @@ __device__ __forceinline__ void hqs_tile_point(const Decode& d, uint32_t lo, uin
         uint32_t ux, ry, dep;
         project_interior(pj, p[0], p[1], p[2], tb.x0, tb.y0, ux, ry, dep);
         if (ux < (uint32_t)tb.tw && ry < (uint32_t)tb.th) {
-            hqs_band_point<PASS>(ux, ry, dep, color, tdepth, tacc, tb);
+            hqs_band_point<PASS>(ux, ry, dep, color, tdepth, tacc, tb, pend);
             return;
         }
     }
@@ -814,7 +845,7 @@ __device__ __forceinline__ void hqs_tile_point(const Decode& d, uint32_t lo, uin
     if (TILE) {
         const uint32_t ux = (uint32_t)(ix - tb.x0), ry = (uint32_t)(iy - tb.y0);
         if (ux < (uint32_t)tb.tw && ry < (uint32_t)tb.th) {
-            hqs_band_point<PASS>(ux, ry, dep, color, tdepth, tacc, tb);
+            hqs_band_point<PASS>(ux, ry, dep, color, tdepth, tacc, tb, pend);
             return;
         }
         if (!tb.first) return;
@@ -843,14 +874,16 @@ __device__ __forceinline__ void hqs_tile_loop(const Decode& d, const uint32_t* _
         const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
         const u32x4 col = PASS == 1 ? ld_plane<NT>(c4 + q) : u32x4{0, 0, 0, 0};
         const uint32_t base = q * 4u - lead;
+        PendingSums pend;
         hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.x, m.x, c.x, col.x, base + 0, np, pj, depth, acc, tdepth,
-                                                 tacc, tb);
+                                                 tacc, tb, pend);
         hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.y, m.y, c.y, col.y, base + 1, np, pj, depth, acc, tdepth,
-                                                 tacc, tb);
+                                                 tacc, tb, pend);
         hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.z, m.z, c.z, col.z, base + 2, np, pj, depth, acc, tdepth,
-                                                 tacc, tb);
+                                                 tacc, tb, pend);
         hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.w, m.w, c.w, col.w, base + 3, np, pj, depth, acc, tdepth,
-                                                 tacc, tb);
+                                                 tacc, tb, pend);
+        if (PASS == 1) pending_flush(pend, tacc);
     }
 }
 
