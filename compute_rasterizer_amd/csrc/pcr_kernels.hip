@@ -772,14 +772,14 @@ struct TileBand {
     bool first;
 };
 
-// Pass 2's sums for one tile pixel, carried in registers over the four
-// points of a thread's quad. The quad's points are consecutive in the
-// batch's Morton order, and in a far batch most of them land on one pixel:
-// a run of them is added to the tile as one pair of ds_add_u64 instead of
-// one pair per point. The LDS performs adds on one word one after the other,
-// so a far batch's adds (hundreds of points per pixel) queued on few words.
-// At most 4 x 255 per channel is held in a register, so a half never carries
-// into the other, and the tile word gets the same sum.
+// Pass 2's sums for one tile pixel, carried in registers over the eight
+// points of a thread's two quads. The points are consecutive in the batch's
+// Morton order, and in a far batch most of them land on one pixel: a run of
+// them is added to the tile as one pair of ds_add_u64 instead of one pair
+// per point. The LDS performs adds on one word one after the other, so a far
+// batch's adds (hundreds of points per pixel) queued on few words. At most
+// 8 x 255 per channel is held in a register, so a half never carries into
+// the other, and the tile word gets the same sum.
 struct PendingSums {
     uint32_t t = 0xFFFFFFFFu;          // tile pixel, or none
     unsigned long long rg = 0, bc = 0; // R | G << 32, B | count << 32
@@ -868,21 +868,38 @@ __device__ __forceinline__ void hqs_tile_loop(const Decode& d, const uint32_t* _
     const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
     const u32x4* c4 = reinterpret_cast<const u32x4*>(rgba + (first - lead));
     const uint32_t nq = (np + lead + 3u) >> 2;
-    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
-        const u32x4 a = ld_plane<NT>(lo4 + q);
-        const u32x4 m = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
-        const u32x4 c = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
-        const u32x4 col = PASS == 1 ? ld_plane<NT>(c4 + q) : u32x4{0, 0, 0, 0};
-        const uint32_t base = q * 4u - lead;
+    // Pass 2 takes two adjacent quads per thread, so that a run of one
+    // pixel's points (PendingSums) spans up to eight consecutive points. At
+    // the default 2048-px tile its LDS (40 KiB per workgroup) allows four
+    // workgroups per CU, which leaves the registers for the second quad.
+    // The second quad of a pair past the batch's last is not read; its
+    // points have local >= np.
+    constexpr uint32_t QPT = PASS == 1 ? 2u : 1u;
+    const u32x4 z = u32x4{0, 0, 0, 0};
+    for (uint32_t q0 = threadIdx.x * QPT; q0 < nq; q0 += BLOCK * QPT) {
+        u32x4 a[QPT], m[QPT], c[QPT], col[QPT];
+#pragma unroll
+        for (uint32_t k = 0; k < QPT; k++) {
+            const uint32_t q = q0 + k;
+            const bool in = k == 0 || q < nq;
+            a[k] = in ? ld_plane<NT>(lo4 + q) : z;
+            m[k] = LC <= 1 && in ? ld_plane<NT>(med4 + q) : z;
+            c[k] = LC == 0 && in ? ld_plane<NT>(hi4 + q) : z;
+            col[k] = PASS == 1 && in ? ld_plane<NT>(c4 + q) : z;
+        }
         PendingSums pend;
-        hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.x, m.x, c.x, col.x, base + 0, np, pj, depth, acc, tdepth,
-                                                 tacc, tb, pend);
-        hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.y, m.y, c.y, col.y, base + 1, np, pj, depth, acc, tdepth,
-                                                 tacc, tb, pend);
-        hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.z, m.z, c.z, col.z, base + 2, np, pj, depth, acc, tdepth,
-                                                 tacc, tb, pend);
-        hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a.w, m.w, c.w, col.w, base + 3, np, pj, depth, acc, tdepth,
-                                                 tacc, tb, pend);
+#pragma unroll
+        for (uint32_t k = 0; k < QPT; k++) {
+            const uint32_t base = (q0 + k) * 4u - lead;
+            hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a[k].x, m[k].x, c[k].x, col[k].x, base + 0, np, pj, depth,
+                                                     acc, tdepth, tacc, tb, pend);
+            hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a[k].y, m[k].y, c[k].y, col[k].y, base + 1, np, pj, depth,
+                                                     acc, tdepth, tacc, tb, pend);
+            hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a[k].z, m[k].z, c[k].z, col[k].z, base + 2, np, pj, depth,
+                                                     acc, tdepth, tacc, tb, pend);
+            hqs_tile_point<PASS, LC, TILE, INTERIOR>(d, a[k].w, m[k].w, c[k].w, col[k].w, base + 3, np, pj, depth,
+                                                     acc, tdepth, tacc, tb, pend);
+        }
         if (PASS == 1) pending_flush(pend, tacc);
     }
 }
