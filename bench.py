@@ -785,7 +785,10 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
             "bytes_per_launch_mean": int(mean("raster_bytes")),
-            "bytes_read_per_launch_mean": int(mean("raster_bytes_read"))}
+            "bytes_read_per_launch_mean": int(mean("raster_bytes_read")),
+            # framebuffer atomicMin rate: atomics the kernel issued (its own count, per frame) over the
+            # timed raster time of the same views
+            "atomics_G_per_s": round(sum(cnt[i]["atomics"] for i in idx) / (raster_ms.sum() / 1e3) / 1e9, 2)}
     prof = measured_traffic(kernel, config)
     if prof:
         roof["traffic"] = int(prof["traffic_bytes_per_launch"])
