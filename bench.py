@@ -649,7 +649,7 @@ def cpu_baseline(ctx, cloud, views, gpu_fb, prm, frames, n_points):
     and the median of `frames` frames on one thread."""
     from oracle import orc
     avail = len(os.sched_getaffinity(0))
-    cores = min(avail, 16)   # a one-GPU box gives a job 16 host cores, whatever the node has
+    cores = min(avail, 16)   # a one-GPU job gets a 16-core share of the host, whatever the node has
     lay = cloud.download()
     b, lo, med, hi = lay["batches"], lay["lo"], lay["med"], lay["hi"]
 
@@ -720,7 +720,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py: this build renders on one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not "
+        print(f"bench.py: the renderer runs on one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not "
               f"supported (sharding batches across GPUs with a min reduce-scatter of the framebuffer is "
               f"designed, not built).", file=sys.stderr)
         sys.exit(2)

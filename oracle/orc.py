@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle liborc.so (TEST INFRASTRUCTURE).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module, and only as the checker. It restates the reference's per-frame
+this module, and only to check the GPU's results against. It restates the reference's per-frame
 shader logic in plain C (pcr_oracle.c cites the reference file:line of each
 function).
 """

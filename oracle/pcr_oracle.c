@@ -1,6 +1,6 @@
 /*
  * pcr_oracle.c — CPU oracle (TEST INFRASTRUCTURE: only tests/, smoke() and
- * bench.py's cpu_baseline leg load it, as the checker).
+ * bench.py's cpu_baseline leg load it, to check the GPU's results against).
  *
  * Restates, in plain C and a fixed fp32 operation order (see pcr_oracle.h),
  * the reference's per-frame logic:
