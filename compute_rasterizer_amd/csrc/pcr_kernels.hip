@@ -574,9 +574,8 @@ __device__ __forceinline__ uint32_t tiled_batch(uint32_t bi, int level, const pc
 // after the other (with the split halves ordered as in k_raster_tiled, the
 // 1e9-point 100 m flythrough's raster is 4.7% faster than with one item per
 // atomic, and three or four items are no better). take() is called by every
-// thread of the workgroup; the
-// returned item is workgroup-uniform. `s_item` must not be rewritten before
-// the workgroup's next barrier.
+// thread of the workgroup; the returned item is workgroup-uniform. `s_item`
+// must not be rewritten before the workgroup's next barrier.
 struct QueueTake {
     uint32_t next = 0xFFFFFFFFu;   // the second item of the last pair, not yet drawn
     __device__ __forceinline__ uint32_t take(uint32_t* head, uint32_t& s_item) {
