@@ -1058,6 +1058,130 @@ __device__ __forceinline__ uint32_t hqs_tiled_batch(uint32_t bi, int level, cons
     return 0;
 }
 
+// hqs pass 1 of one batch for a pair of views at once, the next two views a
+// multi-view launch draws the batch for, at one precision level: each point
+// is decoded once and projected for both views, into two depth tiles. Used
+// when both views' rectangles are interior (screen_rect) and fit the tile in
+// one band; the caller draws the views one at a time otherwise. Per view the
+// operations are those of hqs_tiled_batch's pass 1 (hqs_tile_point with one
+// band), so each depth buffer is the same; each view keeps its own
+// occlusion test.
+
+// The post-decode half of hqs_tile_point for pass 1 of an interior batch
+// with one band: the tile pixel's depth, or the direct path for a point that
+// rounding puts outside the rectangle.
+__device__ __forceinline__ void hqs1_view_point(const float p[3], const Proj& pj, uint32_t* __restrict__ depth,
+                                                uint32_t* tdepth, int x0, int y0, int tw, int th) {
+    uint32_t ux, uy, dep;
+    project_interior(pj, p[0], p[1], p[2], x0, y0, ux, uy, dep);
+    if (ux < (uint32_t)tw && uy < (uint32_t)th) {
+        lds_min32(tdepth + uy * (uint32_t)tw + ux, dep);
+        return;
+    }
+    int ix, iy;
+    if (!project_xy(pj, p[0], p[1], p[2], ix, iy, dep)) return;
+    const uint32_t vx = (uint32_t)(ix - x0), vy = (uint32_t)(iy - y0);
+    if (vx < (uint32_t)tw && vy < (uint32_t)th) {
+        lds_min32(tdepth + vy * (uint32_t)tw + vx, dep);
+        return;
+    }
+    hqs_depth(depth, (uint32_t)ix + (uint32_t)iy * (uint32_t)pj.w, dep);
+}
+
+// One view of a pair: its projection, buffers, rectangle and whether it draws.
+struct PairView {
+    Proj pj;
+    uint32_t* depth;
+    uint32_t* tdepth;
+    int x0, y0, tw, th;
+    bool draw;
+};
+
+template <int BLOCK, int LC, bool NT>
+__device__ __forceinline__ void hqs1_pair_loop(const Decode& d, const uint32_t* __restrict__ lo,
+                                               const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                               uint32_t first, uint32_t np, const PairView& a, const PairView& b) {
+    const uint32_t lead = first & 3u;   // see tile_loop()
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
+    const uint32_t nq = (np + lead + 3u) >> 2;
+    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
+        const u32x4 w0 = ld_plane<NT>(lo4 + q);
+        const u32x4 w1 = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 w2 = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
+        const uint32_t base = q * 4u - lead;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (base + (uint32_t)j >= np) continue;
+            float p[3];
+            decode<LC>(d, w0[j], w1[j], w2[j], p);
+            if (a.draw) hqs1_view_point(p, a.pj, a.depth, a.tdepth, a.x0, a.y0, a.tw, a.th);
+            if (b.draw) hqs1_view_point(p, b.pj, b.depth, b.tdepth, b.x0, b.y0, b.tw, b.th);
+        }
+    }
+}
+
+// Returns false, having done nothing, when the pair cannot be drawn together.
+template <int BLOCK, int TILE_PIX, bool NT>
+__device__ __forceinline__ bool hqs1_pair_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
+                                                const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
+                                                const uint32_t* __restrict__ hi, const pcr_view& va,
+                                                uint32_t* __restrict__ depth_a, const pcr_view& vb,
+                                                uint32_t* __restrict__ depth_b, uint32_t* tdepth_a,
+                                                uint32_t* tdepth_b, bool occl, uint32_t part, uint32_t nparts) {
+    const pcr_batch h = b[bi];
+    pcr_batch hp = h;   // as in hqs_tiled_batch: the points this item draws
+    if (nparts > 1) {
+        const uint32_t k = (h.num_points + nparts - 1) / nparts, off = min(part * k, h.num_points);
+        hp.first_point += off;
+        hp.num_points = min(k, h.num_points - off);
+    }
+    int ax0 = 0, ay0 = 0, ax1 = -1, ay1 = -1, bx0 = 0, by0 = 0, bx1 = -1, by1 = -1;
+    bool ai, bi_;
+    float awlo, bwlo;
+    const bool aon = screen_rect(va.transform, h.min, h.max, va.width, va.height, ax0, ay0, ax1, ay1, &ai, &awlo);
+    const bool bon = screen_rect(vb.transform, h.min, h.max, vb.width, vb.height, bx0, by0, bx1, by1, &bi_, &bwlo);
+    const int atw = ax1 - ax0 + 1, ath = ay1 - ay0 + 1, btw = bx1 - bx0 + 1, bth = by1 - by0 + 1;
+    constexpr int cap = HqsTile<0, TILE_PIX>::kCap;
+    if (!(aon && bon && ai && bi_ && atw >= 2 && btw >= 2 && atw * ath <= cap && btw * bth <= cap &&
+          h.num_points <= kHqsTileMaxPoints))
+        return false;
+    // each view's occlusion test, as in hqs_tiled_batch
+    const bool da = !(occl && awlo > 0.0f && atw * ath <= kOcclusionMaxPix) ||
+                    rect_open<BLOCK, 1>(depth_a, va.width, ax0, ay0, atw, atw * ath, awlo);
+    const bool db = !(occl && bwlo > 0.0f && btw * bth <= kOcclusionMaxPix) ||
+                    rect_open<BLOCK, 1>(depth_b, vb.width, bx0, by0, btw, btw * bth, bwlo);
+    if (!da && !db) return true;
+    const Decode d = make_decode(level, h.min, h.max);
+    const PairView a{make_proj(va.transform, va.width, va.height), depth_a, tdepth_a, ax0, ay0, atw, ath, da};
+    const PairView c{make_proj(vb.transform, vb.width, vb.height), depth_b, tdepth_b, bx0, by0, btw, bth, db};
+    for (int i = threadIdx.x; i < atw * ath; i += BLOCK) tdepth_a[i] = PCR_DEPTH_CLEAR;
+    for (int i = threadIdx.x; i < btw * bth; i += BLOCK) tdepth_b[i] = PCR_DEPTH_CLEAR;
+    __syncthreads();
+    if (level == 0) hqs1_pair_loop<BLOCK, 0, NT>(d, lo, med, hi, hp.first_point, hp.num_points, a, c);
+    else if (level == 1) hqs1_pair_loop<BLOCK, 1, NT>(d, lo, med, hi, hp.first_point, hp.num_points, a, c);
+    else hqs1_pair_loop<BLOCK, 2, NT>(d, lo, med, hi, hp.first_point, hp.num_points, a, c);
+    __syncthreads();
+    // flush each view's touched pixels (row = i / tw as a multiply-high, see tiled_batch)
+    const PairView* vs[2] = {&a, &c};
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const PairView& pv = *vs[e];
+        if (!pv.draw) continue;
+        const int n = pv.tw * pv.th;
+        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)pv.tw - 1) / (uint64_t)pv.tw);
+        const uint32_t width = (uint32_t)pv.pj.w;
+        for (int i = threadIdx.x; i < n; i += BLOCK) {
+            const uint32_t t = pv.tdepth[i];
+            if (t == PCR_DEPTH_CLEAR) continue;
+            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)pv.tw;
+            hqs_depth(pv.depth, (uint32_t)pv.x0 + ux + ((uint32_t)pv.y0 + uy) * width, t);
+        }
+    }
+    return true;
+}
+
 // Persistent workgroups over the select stage's list, in the spread order of
 // k_raster_tiled; pass 1 takes items from queue_head, pass 2 (launched after
 // pass 1 has finished) from queue_head2. Pass 1 with tiles up to 2048 px is
@@ -1172,6 +1296,7 @@ __global__ __launch_bounds__(BLOCK) void k_mv_hqs_tiled(const pcr_batch* __restr
                                                         uint32_t* const* __restrict__ depths,
                                                         uint32_t* const* __restrict__ accs, int occl) {
     __shared__ HqsTile<PASS, TILE_PIX> tile;
+    __shared__ uint32_t tile2[PASS == 0 ? TILE_PIX : 1];   // pass 1: the second view of a pair
     __shared__ uint32_t s_item, s_nv;
     __shared__ uint8_t s_vid[64], s_lv[64];
     uint32_t* head = PASS == 0 ? &st->queue_head : &st->queue_head2;
@@ -1199,12 +1324,25 @@ __global__ __launch_bounds__(BLOCK) void k_mv_hqs_tiled(const pcr_batch* __restr
         }
         __syncthreads();
         const uint32_t nv = __builtin_amdgcn_readfirstlane(s_nv);
-        for (uint32_t k = 0; k < nv; k++) {
+        for (uint32_t k = 0; k < nv;) {
             const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
             const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
+            // pass 1: the next two views at one level in one pass over the points
+            if (PASS == 0 && k + 1 < nv) {
+                const int vj = __builtin_amdgcn_readfirstlane(s_vid[k + 1]);
+                const int lj = __builtin_amdgcn_readfirstlane(s_lv[k + 1]);
+                if (lv == lj && hqs1_pair_batch<BLOCK, TILE_PIX, false>(bi, lv, b, lo, med, hi, views[vi], depths[vi],
+                                                                         views[vj], depths[vj], tile.depth, tile2,
+                                                                         occl != 0, part, nparts)) {
+                    __syncthreads();   // the tiles are reused by the next views
+                    k += 2;
+                    continue;
+                }
+            }
             hqs_tiled_batch<BLOCK, TILE_PIX, PASS, false>(bi, lv, b, lo, med, hi, rgba, views[vi], depths[vi],
                                                            accs[vi], tile.depth, tile.acc, occl != 0, part, nparts);
             __syncthreads();   // the tile is reused by the next view
+            k++;
         }
     }
 }

@@ -93,6 +93,45 @@ def test_multiview_hqs_equals_single_views(ctx, city, use_graph, tune):
         fb.close()
 
 
+def eyes(seed, n_pairs=1, ipd=0.065):
+    """n_pairs stereo pairs (eyes ipd apart) of the city from about 250 m, 1280x720."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(n_pairs):
+        x, y = rng.uniform(600, 1400, 2)
+        eye = np.array([x, y - 700.0, rng.uniform(200, 300)])
+        tgt = np.array([x, y + 300.0, 40.0])
+        for off in (0.0, ipd):
+            r = np.array([off, 0.0, 0.0])
+            views.append(pcr.look_at(tuple(eye + r), tuple(tgt + r), up=(0, 0, 1), near=1.0, far=5000.0,
+                                     width=1280, height=720))
+    return views
+
+
+@pytest.mark.parametrize("n_pairs,tune", [(1, (256, 40, 0, 2048)), (1, (128, 40, 0, 1024)), (2, (256, 40, 0, 4096))])
+def test_view_pairs_pass1_equal_oracle(ctx, city, n_pairs, tune):
+    """Two views of a batch at one level draw hqs pass 1 in one pass over the
+    points (two depth tiles); stereo pairs take that path for most batches.
+    Every view's depths, sums and image equal the oracle's."""
+    cloud, lay = city
+    prm = pcr.params(pcr.MODE_HQS)
+    views = eyes(5 + n_pairs, n_pairs)
+    fbs = [ctx.framebuffer(v.width, v.height, pcr.FB_HQS) for v in views]
+    mv = pcr.MultiView(ctx, cloud, views, fbs, prm, tune=pcr.Tune(*tune))
+    mv.launch()
+    ctx.synchronize()
+    for k, v in enumerate(views):
+        rd, ra = orc.hqs_batched(lay["batches"], lay["lo"], lay["med"], lay["hi"], lay["rgba"], v)
+        gd, ga = fbs[k].hqs()
+        assert (ra[3::4] > 0).sum() > 1000
+        assert np.array_equal(gd, rd), k
+        assert np.array_equal(ga, ra), k
+        assert np.array_equal(fbs[k].rgba(), orc.hqs_resolve(ra, prm.background)), k
+    mv.close()
+    for fb in fbs:
+        fb.close()
+
+
 @pytest.mark.parametrize("rule,window,tune", [(pcr.LOD_SPACING, 1, (128, 40, 0, 0)), (pcr.LOD_SIZE80, 2, (256, 40, 0, 2048)),
                                               (pcr.LOD_SPACING, 0, (128, 40, 0, 1024))])
 def test_multiview_nodes_hqs_equals_oracle(ctx, octree, rule, window, tune):
