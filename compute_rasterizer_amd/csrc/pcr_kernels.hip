@@ -692,6 +692,139 @@ int launch_raster_tiled(hipStream_t s, const pcr_cloud* c, const pcr_view& v, in
 // view; for each view that draws the batch the workgroup runs the tiled
 // batch pass with that view's level. After the first view the batch's planes
 // come from cache, so HBM sees each plane once per frame.
+// One batch for a pair of views at once, the next two views a multi-view
+// launch draws the batch for, at one precision level: each point is decoded
+// once and projected for both views, into the two halves of the LDS tile.
+// Used when both views' rectangles are interior (screen_rect) and fit half
+// the tile; the caller draws the views one at a time otherwise. Per view the
+// operations are those of tiled_batch (tile_point's interior path, the
+// four-pixel flush), so each framebuffer is the same; each view keeps its own
+// occlusion test.
+struct RasterPairView {
+    Proj pj;
+    unsigned long long* fb;
+    unsigned long long* tile;
+    int x0, y0, tw, th;
+    bool draw;
+};
+
+// The post-decode half of tile_point (interior batch, tiled) for one view.
+__device__ __forceinline__ void raster_pair_point(const float p[3], unsigned long long key_low, const RasterPairView& rv,
+                                                  uint32_t* natom) {
+    uint32_t ux, uy, depth;
+    project_interior(rv.pj, p[0], p[1], p[2], rv.x0, rv.y0, ux, uy, depth);
+    if (ux < (uint32_t)rv.tw && uy < (uint32_t)rv.th) {
+        lds_min(rv.tile + __umul24(uy, (uint32_t)rv.tw) + ux, ((unsigned long long)depth << 32) | key_low);
+        return;
+    }
+    int ix, iy;
+    if (!project_xy(rv.pj, p[0], p[1], p[2], ix, iy, depth)) return;
+    const unsigned long long key = ((unsigned long long)depth << 32) | key_low;
+    const uint32_t vx = (uint32_t)(ix - rv.x0), vy = (uint32_t)(iy - rv.y0);
+    if (vx < (uint32_t)rv.tw && vy < (uint32_t)rv.th) lds_min(rv.tile + vy * (uint32_t)rv.tw + vx, key);
+    else if (fb_min(rv.fb, (uint32_t)ix + (uint32_t)iy * (uint32_t)rv.pj.w, key)) lds_add32(natom, 1u);
+}
+
+template <int BLOCK, int LC, bool NT>
+__device__ __forceinline__ void raster_pair_loop(const Decode& d, const uint32_t* __restrict__ lo,
+                                                 const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                                 uint32_t first, uint32_t np, const RasterPairView& a,
+                                                 const RasterPairView& c, uint32_t* natom) {
+    const uint32_t lead = first & 3u;   // see tile_loop()
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
+    const uint32_t nq = (np + lead + 3u) >> 2;
+    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
+        const u32x4 w0 = ld_plane<NT>(lo4 + q);
+        const u32x4 w1 = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 w2 = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
+        const uint32_t base = q * 4u - lead;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t local = base + (uint32_t)j;
+            if (local >= np) continue;
+            float p[3];
+            decode<LC>(d, w0[j], w1[j], w2[j], p);
+            if (a.draw) raster_pair_point(p, first + local, a, natom);
+            if (c.draw) raster_pair_point(p, first + local, c, natom);
+        }
+    }
+}
+
+// Returns false, having done nothing, when the pair cannot be drawn together.
+template <int BLOCK, int TILE_PIX, bool NT>
+__device__ __forceinline__ bool raster_pair_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
+                                                  const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
+                                                  const uint32_t* __restrict__ hi, const pcr_view& va,
+                                                  unsigned long long* __restrict__ fb_a, const pcr_view& vb,
+                                                  unsigned long long* __restrict__ fb_b, unsigned long long* tile,
+                                                  uint32_t* natom, bool occl, uint32_t part, uint32_t nparts) {
+    pcr_batch h = b[bi];
+    if (nparts > 1) {   // as in tiled_batch
+        const uint32_t k = (h.num_points + nparts - 1) / nparts, off = min(part * k, h.num_points);
+        h.first_point += off;
+        h.num_points = min(k, h.num_points - off);
+    }
+    int ax0 = 0, ay0 = 0, ax1 = -1, ay1 = -1, bx0 = 0, by0 = 0, bx1 = -1, by1 = -1;
+    bool ai, bi_;
+    float awlo, bwlo;
+    const bool aon = screen_rect(va.transform, h.min, h.max, va.width, va.height, ax0, ay0, ax1, ay1, &ai, &awlo);
+    const bool bon = screen_rect(vb.transform, h.min, h.max, vb.width, vb.height, bx0, by0, bx1, by1, &bi_, &bwlo);
+    const int atw = ax1 - ax0 + 1, ath = ay1 - ay0 + 1, btw = bx1 - bx0 + 1, bth = by1 - by0 + 1;
+    constexpr int half = TILE_PIX / 2;
+    if (!(aon && bon && ai && bi_ && atw >= 2 && btw >= 2 && atw <= half && ath <= half && btw <= half &&
+          bth <= half && atw * ath <= half && btw * bth <= half))
+        return false;
+    // each view's occlusion test, as in tiled_batch
+    const bool da = !(occl && awlo > 0.0f && atw * ath <= kOcclusionMaxPix) ||
+                    rect_open<BLOCK, 0>(fb_a, va.width, ax0, ay0, atw, atw * ath, awlo);
+    const bool db = !(occl && bwlo > 0.0f && btw * bth <= kOcclusionMaxPix) ||
+                    rect_open<BLOCK, 0>(fb_b, vb.width, bx0, by0, btw, btw * bth, bwlo);
+    if (!da && !db) return true;
+    const Decode d = make_decode(level, h.min, h.max);
+    const RasterPairView a{make_proj(va.transform, va.width, va.height), fb_a, tile, ax0, ay0, atw, ath, da};
+    const RasterPairView c{make_proj(vb.transform, vb.width, vb.height), fb_b, tile + half, bx0, by0, btw, bth, db};
+    for (int i = threadIdx.x; i < atw * ath; i += BLOCK) a.tile[i] = PCR_FB_CLEAR;
+    for (int i = threadIdx.x; i < btw * bth; i += BLOCK) c.tile[i] = PCR_FB_CLEAR;
+    __syncthreads();
+    if (level == 0) raster_pair_loop<BLOCK, 0, NT>(d, lo, med, hi, h.first_point, h.num_points, a, c, natom);
+    else if (level == 1) raster_pair_loop<BLOCK, 1, NT>(d, lo, med, hi, h.first_point, h.num_points, a, c, natom);
+    else raster_pair_loop<BLOCK, 2, NT>(d, lo, med, hi, h.first_point, h.num_points, a, c, natom);
+    __syncthreads();
+    // each view's flush, as in tiled_batch
+    const RasterPairView* vs[2] = {&a, &c};
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const RasterPairView& rv = *vs[e];
+        if (!rv.draw) continue;
+        const int n = rv.tw * rv.th;
+        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)rv.tw - 1) / (uint64_t)rv.tw);
+        const uint32_t width = (uint32_t)rv.pj.w;
+        constexpr int F = 4;
+        for (int i0 = threadIdx.x; i0 < n; i0 += F * BLOCK) {
+            unsigned long long k[F], old[F];
+            uint32_t pix[F];
+#pragma unroll
+            for (int j = 0; j < F; j++) {
+                const int i = i0 + j * BLOCK;
+                k[j] = i < n ? rv.tile[i] : PCR_FB_CLEAR;
+                const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)rv.tw;
+                pix[j] = (uint32_t)rv.x0 + ux + ((uint32_t)rv.y0 + uy) * width;
+            }
+#pragma unroll
+            for (int j = 0; j < F; j++) old[j] = k[j] != PCR_FB_CLEAR ? rv.fb[pix[j]] : 0ull;
+#pragma unroll
+            for (int j = 0; j < F; j++)
+                if (k[j] < old[j]) {
+                    atomicMin(rv.fb + pix[j], k[j]);
+                    lds_add32(natom, 1u);
+                }
+        }
+    }
+    return true;
+}
+
 template <int BLOCK, int TILE_PIX>
 __global__ __launch_bounds__(BLOCK) void k_mv_raster_tiled(const pcr_batch* __restrict__ b, uint32_t nb,
                                                            const uint8_t* __restrict__ levels,
@@ -729,12 +862,25 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster_tiled(const pcr_batch* __re
         }
         __syncthreads();
         const uint32_t nv = __builtin_amdgcn_readfirstlane(s_nv);
-        for (uint32_t k = 0; k < nv; k++) {
+        for (uint32_t k = 0; k < nv;) {
             const int vi = __builtin_amdgcn_readfirstlane(s_vid[k]);
             const int lv = __builtin_amdgcn_readfirstlane(s_lv[k]);
+            // the next two views at one level in one pass over the points
+            if (k + 1 < nv) {
+                const int vj = __builtin_amdgcn_readfirstlane(s_vid[k + 1]);
+                const int lj = __builtin_amdgcn_readfirstlane(s_lv[k + 1]);
+                if (lv == lj && raster_pair_batch<BLOCK, TILE_PIX, false>(bi, lv, b, lo, med, hi, views[vi], fbs[vi],
+                                                                           views[vj], fbs[vj], tile, &s_natom,
+                                                                           occl != 0, part, nparts)) {
+                    __syncthreads();   // the tile is reused by the next views
+                    k += 2;
+                    continue;
+                }
+            }
             tiled_batch<BLOCK, TILE_PIX, false>(bi, lv, b, lo, med, hi, views[vi], fbs[vi], tile, &s_natom,
                                                 occl != 0, part, nparts);
             __syncthreads();   // the tile is reused by the next view
+            k++;
         }
     }
 }

@@ -81,6 +81,46 @@ def test_multiview_equals_single_views(ctx, city, use_graph, tune):
     mv.close()
 
 
+def eye_pairs(seed, n_pairs, ipd=0.065):
+    """n_pairs stereo pairs (eyes ipd apart) of the city from about 250 m, 1280x720."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(n_pairs):
+        x, y = rng.uniform(600, 1400, 2)
+        eye = np.array([x, y - 700.0, rng.uniform(200, 300)])
+        tgt = np.array([x, y + 300.0, 40.0])
+        for off in (0.0, ipd):
+            r = np.array([off, 0.0, 0.0])
+            views.append(pcr.look_at(tuple(eye + r), tuple(tgt + r), up=(0, 0, 1), near=1.0, far=5000.0,
+                                     width=1280, height=720))
+    return views
+
+
+@pytest.mark.parametrize("occl", [True, False])
+@pytest.mark.parametrize("n_pairs,tune", [(1, (256, 40, 0, 2048)), (1, (128, 40, 0, 1024)), (2, (256, 40, 0, 4096))])
+def test_view_pairs_equal_oracle(ctx, city, n_pairs, tune, occl):
+    """Two adjacent views that draw a batch at one level take it in one pass
+    over the points (the two halves of the LDS tile); stereo pairs take that
+    path for most batches. Every view's keys equal the oracle's."""
+    prm = pcr.params(pcr.MODE_LAS2, occlusion_skip=occl)
+    views = eye_pairs(11 + n_pairs, n_pairs)
+    fbs = [ctx.framebuffer(v.width, v.height) for v in views]
+    mv = pcr.MultiView(ctx, city, views, fbs, prm, tune=pcr.Tune(*tune))
+    mv.launch()
+    ctx.synchronize()
+    lay = city.download()
+    for k, v in enumerate(views):
+        ref, _ = orc.render_batched(lay["batches"], lay["lo"], lay["med"], lay["hi"], v, 1)
+        keys = fbs[k].keys()
+        assert (ref != pcr.FB_CLEAR).sum() > 1000
+        assert np.array_equal(keys, ref), (k, int((keys != ref).sum()))
+        kk, img = single_view(ctx, city, v, prm)
+        assert np.array_equal(fbs[k].rgba(), img), k
+    mv.close()
+    for fb in fbs:
+        fb.close()
+
+
 def test_multiview_lod_nodes(ctx, nodes):
     prm = pcr.params(pcr.MODE_NODES, lod_px=1.0)
     views = views_for(SIZES, seed=7)
