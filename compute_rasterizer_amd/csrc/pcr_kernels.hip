@@ -1328,6 +1328,150 @@ __device__ __forceinline__ bool hqs1_pair_batch(uint32_t bi, int level, const pc
     return true;
 }
 
+// hqs pass 2 of one batch for a pair of views at once, under the same
+// conditions as hqs1_pair_batch but with each view's rectangle fitting half
+// the pass-2 tile: the two halves hold each view's final depths and sums.
+// Per view the operations are those of hqs_tiled_batch's pass 2 with one band
+// (hqs_tile_point, the run merge of PendingSums, the add_pairs flush), so
+// each view's accumulators are the same; each view keeps its own occlusion
+// test.
+struct Pair2View {
+    Proj pj;
+    uint32_t* depth;
+    uint32_t* acc;
+    uint32_t* tdepth;
+    unsigned long long* tacc;
+    TileBand tb;
+    bool draw;
+};
+
+// The post-decode half of hqs_tile_point for pass 2 of an interior batch with
+// one band.
+__device__ __forceinline__ void hqs2_view_point(const float p[3], uint32_t color, const Pair2View& pv,
+                                                PendingSums& pend) {
+    uint32_t ux, ry, dep;
+    project_interior(pv.pj, p[0], p[1], p[2], pv.tb.x0, pv.tb.y0, ux, ry, dep);
+    if (ux < (uint32_t)pv.tb.tw && ry < (uint32_t)pv.tb.th) {
+        hqs_band_point<1>(ux, ry, dep, color, pv.tdepth, pv.tacc, pv.tb, pend);
+        return;
+    }
+    int ix, iy;
+    if (!project_xy(pv.pj, p[0], p[1], p[2], ix, iy, dep)) return;
+    const uint32_t vx = (uint32_t)(ix - pv.tb.x0), vy = (uint32_t)(iy - pv.tb.y0);
+    if (vx < (uint32_t)pv.tb.tw && vy < (uint32_t)pv.tb.th) {
+        hqs_band_point<1>(vx, vy, dep, color, pv.tdepth, pv.tacc, pv.tb, pend);
+        return;
+    }
+    hqs_accum(pv.depth, pv.acc, (uint32_t)ix + (uint32_t)iy * (uint32_t)pv.pj.w, dep, color);
+}
+
+template <int BLOCK, int LC, bool NT>
+__device__ __forceinline__ void hqs2_pair_loop(const Decode& d, const uint32_t* __restrict__ lo,
+                                               const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
+                                               const uint32_t* __restrict__ rgba, uint32_t first, uint32_t np,
+                                               const Pair2View& a, const Pair2View& c) {
+    const uint32_t lead = first & 3u;   // see tile_loop()
+    const u32x4* lo4 = reinterpret_cast<const u32x4*>(lo + (first - lead));
+    const u32x4* med4 = reinterpret_cast<const u32x4*>(med + (first - lead));
+    const u32x4* hi4 = reinterpret_cast<const u32x4*>(hi + (first - lead));
+    const u32x4* c4 = reinterpret_cast<const u32x4*>(rgba + (first - lead));
+    const uint32_t nq = (np + lead + 3u) >> 2;
+    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
+        const u32x4 w0 = ld_plane<NT>(lo4 + q);
+        const u32x4 w1 = LC <= 1 ? ld_plane<NT>(med4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 w2 = LC == 0 ? ld_plane<NT>(hi4 + q) : u32x4{0, 0, 0, 0};
+        const u32x4 col = ld_plane<NT>(c4 + q);
+        const uint32_t base = q * 4u - lead;
+        PendingSums pa, pc;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (base + (uint32_t)j >= np) continue;
+            float p[3];
+            decode<LC>(d, w0[j], w1[j], w2[j], p);
+            if (a.draw) hqs2_view_point(p, col[j], a, pa);
+            if (c.draw) hqs2_view_point(p, col[j], c, pc);
+        }
+        pending_flush(pa, a.tacc);
+        pending_flush(pc, c.tacc);
+    }
+}
+
+// Returns false, having done nothing, when the pair cannot be drawn together.
+template <int BLOCK, int TILE_PIX, bool NT>
+__device__ __forceinline__ bool hqs2_pair_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
+                                                const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
+                                                const uint32_t* __restrict__ hi, const uint32_t* __restrict__ rgba,
+                                                const pcr_view& va, uint32_t* __restrict__ depth_a,
+                                                uint32_t* __restrict__ acc_a, const pcr_view& vb,
+                                                uint32_t* __restrict__ depth_b, uint32_t* __restrict__ acc_b,
+                                                uint32_t* tdepth, unsigned long long* tacc, bool occl, uint32_t part,
+                                                uint32_t nparts) {
+    const pcr_batch h = b[bi];
+    pcr_batch hp = h;   // as in hqs_tiled_batch: the points this item draws
+    if (nparts > 1) {
+        const uint32_t k = (h.num_points + nparts - 1) / nparts, off = min(part * k, h.num_points);
+        hp.first_point += off;
+        hp.num_points = min(k, h.num_points - off);
+    }
+    int ax0 = 0, ay0 = 0, ax1 = -1, ay1 = -1, bx0 = 0, by0 = 0, bx1 = -1, by1 = -1;
+    bool ai, bi_;
+    float awlo, bwlo;
+    const bool aon = screen_rect(va.transform, h.min, h.max, va.width, va.height, ax0, ay0, ax1, ay1, &ai, &awlo);
+    const bool bon = screen_rect(vb.transform, h.min, h.max, vb.width, vb.height, bx0, by0, bx1, by1, &bi_, &bwlo);
+    const int atw = ax1 - ax0 + 1, ath = ay1 - ay0 + 1, btw = bx1 - bx0 + 1, bth = by1 - by0 + 1;
+    constexpr int half = HqsTile<1, TILE_PIX>::kCap / 2;
+    if (!(aon && bon && ai && bi_ && atw >= 2 && btw >= 2 && atw * ath <= half && btw * bth <= half &&
+          h.num_points <= kHqsTileMaxPoints))
+        return false;
+    // each view's occlusion test, as in hqs_tiled_batch's pass 2
+    const bool da = !(occl && awlo > 0.0f && atw * ath <= kOcclusionMaxPix) ||
+                    rect_open<BLOCK, 2>(depth_a, va.width, ax0, ay0, atw, atw * ath, awlo);
+    const bool db = !(occl && bwlo > 0.0f && btw * bth <= kOcclusionMaxPix) ||
+                    rect_open<BLOCK, 2>(depth_b, vb.width, bx0, by0, btw, btw * bth, bwlo);
+    if (!da && !db) return true;
+    const Decode d = make_decode(level, h.min, h.max);
+    const Pair2View a{make_proj(va.transform, va.width, va.height), depth_a, acc_a, tdepth, tacc,
+                      TileBand{ax0, ay0, atw, ath, 0, ath, true}, da};
+    const Pair2View c{make_proj(vb.transform, vb.width, vb.height), depth_b, acc_b, tdepth + half, tacc + 2 * half,
+                      TileBand{bx0, by0, btw, bth, 0, bth, true}, db};
+    const Pair2View* vs[2] = {&a, &c};
+    // each drawing view's final depths into its half, its sums cleared
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const Pair2View& pv = *vs[e];
+        if (!pv.draw) continue;
+        const int n = pv.tb.tw * pv.tb.th;
+        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)pv.tb.tw - 1) / (uint64_t)pv.tb.tw);
+        for (int i = threadIdx.x; i < n; i += BLOCK) {
+            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)pv.tb.tw;
+            pv.tdepth[i] = pv.depth[(uint32_t)pv.tb.x0 + ux + ((uint32_t)pv.tb.y0 + uy) * (uint32_t)pv.pj.w];
+            pv.tacc[2 * i] = 0ull;
+            pv.tacc[2 * i + 1] = 0ull;
+        }
+    }
+    __syncthreads();
+    if (level == 0) hqs2_pair_loop<BLOCK, 0, NT>(d, lo, med, hi, rgba, hp.first_point, hp.num_points, a, c);
+    else if (level == 1) hqs2_pair_loop<BLOCK, 1, NT>(d, lo, med, hi, rgba, hp.first_point, hp.num_points, a, c);
+    else hqs2_pair_loop<BLOCK, 2, NT>(d, lo, med, hi, rgba, hp.first_point, hp.num_points, a, c);
+    __syncthreads();
+    // flush each view's touched pixels (row = i / tw as a multiply-high, see tiled_batch)
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const Pair2View& pv = *vs[e];
+        if (!pv.draw) continue;
+        const int n = pv.tb.tw * pv.tb.th;
+        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)pv.tb.tw - 1) / (uint64_t)pv.tb.tw);
+        for (int i = threadIdx.x; i < n; i += BLOCK) {
+            const unsigned long long rg = pv.tacc[2 * i], bc = pv.tacc[2 * i + 1];
+            if ((bc >> 32) == 0ull) continue;
+            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)pv.tb.tw;
+            const uint32_t pix = (uint32_t)pv.tb.x0 + ux + ((uint32_t)pv.tb.y0 + uy) * (uint32_t)pv.pj.w;
+            add_pairs(pv.acc + 4ull * pix, (uint32_t)rg, (uint32_t)(rg >> 32), (uint32_t)bc, (uint32_t)(bc >> 32));
+        }
+    }
+    return true;
+}
+
 // Persistent workgroups over the select stage's list, in the spread order of
 // k_raster_tiled; pass 1 takes items from queue_head, pass 2 (launched after
 // pass 1 has finished) from queue_head2. Pass 1 with tiles up to 2048 px is
@@ -1483,6 +1627,20 @@ __global__ __launch_bounds__(BLOCK) void k_mv_hqs_tiled(const pcr_batch* __restr
                     __syncthreads();   // the tiles are reused by the next views
                     k += 2;
                     continue;
+                }
+            }
+            if constexpr (PASS == 1) {
+                if (k + 1 < nv) {   // pass 2: the next two views at one level, each in half the tile
+                    const int vj = __builtin_amdgcn_readfirstlane(s_vid[k + 1]);
+                    const int lj = __builtin_amdgcn_readfirstlane(s_lv[k + 1]);
+                    if (lv == lj && hqs2_pair_batch<BLOCK, TILE_PIX, false>(bi, lv, b, lo, med, hi, rgba, views[vi],
+                                                                             depths[vi], accs[vi], views[vj],
+                                                                             depths[vj], accs[vj], tile.depth,
+                                                                             tile.acc, occl != 0, part, nparts)) {
+                        __syncthreads();   // the tile is reused by the next views
+                        k += 2;
+                        continue;
+                    }
                 }
             }
             hqs_tiled_batch<BLOCK, TILE_PIX, PASS, false>(bi, lv, b, lo, med, hi, rgba, views[vi], depths[vi],

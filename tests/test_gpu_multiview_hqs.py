@@ -109,10 +109,11 @@ def eyes(seed, n_pairs=1, ipd=0.065):
 
 
 @pytest.mark.parametrize("n_pairs,tune", [(1, (256, 40, 0, 2048)), (1, (128, 40, 0, 1024)), (2, (256, 40, 0, 4096))])
-def test_view_pairs_pass1_equal_oracle(ctx, city, n_pairs, tune):
-    """Two views of a batch at one level draw hqs pass 1 in one pass over the
-    points (two depth tiles); stereo pairs take that path for most batches.
-    Every view's depths, sums and image equal the oracle's."""
+def test_view_pairs_equal_oracle(ctx, city, n_pairs, tune):
+    """Two views of a batch at one level draw each hqs pass in one pass over
+    the points (pass 1: two depth tiles; pass 2: the two halves of the pass-2
+    tile); stereo pairs take those paths for most batches. Every view's
+    depths, sums and image equal the oracle's."""
     cloud, lay = city
     prm = pcr.params(pcr.MODE_HQS)
     views = eyes(5 + n_pairs, n_pairs)
