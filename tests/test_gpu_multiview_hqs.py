@@ -93,30 +93,39 @@ def test_multiview_hqs_equals_single_views(ctx, city, use_graph, tune):
         fb.close()
 
 
-def eyes(seed, n_pairs=1, ipd=0.065):
-    """n_pairs stereo pairs (eyes ipd apart) of the city from about 250 m, 1280x720."""
+def eyes(seed, n_pairs=1, ipd=0.065, sizes=((1280, 720), (1280, 720))):
+    """n_pairs stereo pairs (eyes ipd apart) of the city from about 250 m, the
+    two eyes at sizes[0] and sizes[1]."""
     rng = np.random.default_rng(seed)
     views = []
     for _ in range(n_pairs):
         x, y = rng.uniform(600, 1400, 2)
         eye = np.array([x, y - 700.0, rng.uniform(200, 300)])
         tgt = np.array([x, y + 300.0, 40.0])
-        for off in (0.0, ipd):
+        for off, (w, h) in zip((0.0, ipd), sizes):
             r = np.array([off, 0.0, 0.0])
             views.append(pcr.look_at(tuple(eye + r), tuple(tgt + r), up=(0, 0, 1), near=1.0, far=5000.0,
-                                     width=1280, height=720))
+                                     width=w, height=h))
     return views
 
 
-@pytest.mark.parametrize("n_pairs,tune", [(1, (256, 40, 0, 2048)), (1, (128, 40, 0, 1024)), (2, (256, 40, 0, 4096))])
-def test_view_pairs_equal_oracle(ctx, city, n_pairs, tune):
+SAME, MIXED = ((1280, 720), (1280, 720)), ((1280, 720), (1920, 1080))
+
+
+@pytest.mark.parametrize("n_pairs,tune,sizes,occl", [(1, (256, 40, 0, 2048), SAME, True),
+                                                     (1, (128, 40, 0, 1024), SAME, True),
+                                                     (2, (256, 40, 0, 4096), SAME, True),
+                                                     (1, (256, 40, 0, 2048), MIXED, True),
+                                                     (1, (256, 40, 0, 2048), SAME, False)])
+def test_view_pairs_equal_oracle(ctx, city, n_pairs, tune, sizes, occl):
     """Two views of a batch at one level draw each hqs pass in one pass over
     the points (pass 1: two depth tiles; pass 2: the two halves of the pass-2
-    tile); stereo pairs take those paths for most batches. Every view's
-    depths, sums and image equal the oracle's."""
+    tile); stereo pairs take those paths for most batches, also when the two
+    views differ in size. Every view's depths, sums and image equal the
+    oracle's, with and without the occlusion skip."""
     cloud, lay = city
-    prm = pcr.params(pcr.MODE_HQS)
-    views = eyes(5 + n_pairs, n_pairs)
+    prm = pcr.params(pcr.MODE_HQS, occlusion_skip=occl)
+    views = eyes(5 + n_pairs, n_pairs, sizes=sizes)
     fbs = [ctx.framebuffer(v.width, v.height, pcr.FB_HQS) for v in views]
     mv = pcr.MultiView(ctx, cloud, views, fbs, prm, tune=pcr.Tune(*tune))
     mv.launch()

@@ -81,29 +81,34 @@ def test_multiview_equals_single_views(ctx, city, use_graph, tune):
     mv.close()
 
 
-def eye_pairs(seed, n_pairs, ipd=0.065):
-    """n_pairs stereo pairs (eyes ipd apart) of the city from about 250 m, 1280x720."""
+def eye_pairs(seed, n_pairs, ipd=0.065, sizes=((1280, 720), (1280, 720))):
+    """n_pairs stereo pairs (eyes ipd apart) of the city from about 250 m, the
+    two eyes at sizes[0] and sizes[1]."""
     rng = np.random.default_rng(seed)
     views = []
     for _ in range(n_pairs):
         x, y = rng.uniform(600, 1400, 2)
         eye = np.array([x, y - 700.0, rng.uniform(200, 300)])
         tgt = np.array([x, y + 300.0, 40.0])
-        for off in (0.0, ipd):
+        for off, (w, h) in zip((0.0, ipd), sizes):
             r = np.array([off, 0.0, 0.0])
             views.append(pcr.look_at(tuple(eye + r), tuple(tgt + r), up=(0, 0, 1), near=1.0, far=5000.0,
-                                     width=1280, height=720))
+                                     width=w, height=h))
     return views
 
 
 @pytest.mark.parametrize("occl", [True, False])
-@pytest.mark.parametrize("n_pairs,tune", [(1, (256, 40, 0, 2048)), (1, (128, 40, 0, 1024)), (2, (256, 40, 0, 4096))])
-def test_view_pairs_equal_oracle(ctx, city, n_pairs, tune, occl):
+@pytest.mark.parametrize("n_pairs,tune,sizes", [(1, (256, 40, 0, 2048), ((1280, 720), (1280, 720))),
+                                               (1, (128, 40, 0, 1024), ((1280, 720), (1280, 720))),
+                                               (2, (256, 40, 0, 4096), ((1280, 720), (1280, 720))),
+                                               (1, (256, 40, 0, 2048), ((1280, 720), (1920, 1080)))])
+def test_view_pairs_equal_oracle(ctx, city, n_pairs, tune, sizes, occl):
     """Two adjacent views that draw a batch at one level take it in one pass
     over the points (the two halves of the LDS tile); stereo pairs take that
-    path for most batches. Every view's keys equal the oracle's."""
+    path for most batches, also when the two views differ in size. Every
+    view's keys equal the oracle's."""
     prm = pcr.params(pcr.MODE_LAS2, occlusion_skip=occl)
-    views = eye_pairs(11 + n_pairs, n_pairs)
+    views = eye_pairs(11 + n_pairs, n_pairs, sizes=sizes)
     fbs = [ctx.framebuffer(v.width, v.height) for v in views]
     mv = pcr.MultiView(ctx, city, views, fbs, prm, tune=pcr.Tune(*tune))
     mv.launch()
