@@ -182,7 +182,8 @@ __global__ __launch_bounds__(kSelectBlock) void k_select(const pcr_batch* __rest
 #pragma unroll
     for (int L = 0; L < 5; L++) cnt[L] = (uint32_t)__popcll(__ballot(level == L));
     const unsigned long long pts = wave_sum_u64(np);
-    const unsigned long long bytes = wave_sum_u64(keep ? (unsigned long long)np * 4ull * planes_for_level(level) : 0ull);
+    const unsigned long long bytes =
+        wave_sum_u64(keep ? (unsigned long long)np * 4ull * planes_for_level(level) : 0ull);
     if (lane == 0) {
         s_keep[wave] = (uint32_t)__popcll(m);
         s_stat[0][wave] = culled;
@@ -494,6 +495,44 @@ __device__ __forceinline__ bool rect_open(const void* __restrict__ buf, int widt
     return __syncthreads_or(open) != 0;
 }
 
+// Flushes the touched pixels i < n of a 64-bit LDS tile over the rectangle
+// at (x0, y0), tw pixels wide, into the framebuffer: four pixels per thread
+// at a time (fb_min's compare and atomic, reordered across pixels). The
+// compiler keeps each compare load in the branch of its pixel, with a wait
+// after it; issuing the four unconditionally, so that they are in flight
+// together, measured the same: the other waves of the SIMD already cover the
+// wait. row = i / tw as a multiply-high by magic = ceil(2^32 / tw): exact
+// because i < 2^13 and 2 <= tw <= 2^13 keep the error below 1/tw.
+template <int BLOCK>
+__device__ __forceinline__ void tile_flush(const unsigned long long* tile, unsigned long long* __restrict__ fb, int x0,
+                                           int y0, int tw, int n, uint32_t magic, uint32_t width, uint32_t* natom) {
+    constexpr int F = 4;
+    for (int i0 = threadIdx.x; i0 < n; i0 += F * BLOCK) {
+        unsigned long long k[F], old[F];
+        uint32_t pix[F];
+#pragma unroll
+        for (int j = 0; j < F; j++) {
+            const int i = i0 + j * BLOCK;
+            k[j] = i < n ? tile[i] : PCR_FB_CLEAR;
+            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
+            pix[j] = (uint32_t)x0 + ux + ((uint32_t)y0 + uy) * width;
+        }
+#pragma unroll
+        for (int j = 0; j < F; j++) old[j] = k[j] != PCR_FB_CLEAR ? fb[pix[j]] : 0ull;
+#pragma unroll
+        for (int j = 0; j < F; j++)
+            if (k[j] < old[j]) {
+                atomicMin(fb + pix[j], k[j]);
+                lds_add32(natom, 1u);
+            }
+    }
+}
+
+// ceil(2^32 / tw) for the rows of a rectangle tw pixels wide, 2 <= tw.
+__device__ __forceinline__ uint32_t row_magic(int tw) {
+    return (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
+}
+
 // One batch of the tiled raster; `bi` and `level` are workgroup-uniform.
 // Returns the points skipped by the occlusion test (0 if the batch drew).
 template <int BLOCK, int TILE_PIX, bool NT>
@@ -534,34 +573,7 @@ __device__ __forceinline__ uint32_t tiled_batch(uint32_t bi, int level, const pc
         __syncthreads();
         tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, pj, fb, tile, x0, y0, tw, th, natom, interior);
         __syncthreads();
-        // Four touched pixels per thread at a time (fb_min's compare and
-        // atomic, reordered across pixels). The compiler keeps each compare
-        // load in the branch of its pixel, with a wait after it; issuing the
-        // four unconditionally, so that they are in flight together, measured
-        // the same: the other waves of the SIMD already cover the wait.
-        // row = i / tw as a multiply-high by ceil(2^32 / tw): exact because
-        // i < 2^13 and 2 <= tw <= 2^13 keep the error below 1/tw
-        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
-        constexpr int F = 4;
-        for (int i0 = threadIdx.x; i0 < n; i0 += F * BLOCK) {
-            unsigned long long k[F], old[F];
-            uint32_t pix[F];
-#pragma unroll
-            for (int j = 0; j < F; j++) {
-                const int i = i0 + j * BLOCK;
-                k[j] = i < n ? tile[i] : PCR_FB_CLEAR;
-                const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
-                pix[j] = (uint32_t)x0 + ux + ((uint32_t)y0 + uy) * (uint32_t)v.width;
-            }
-#pragma unroll
-            for (int j = 0; j < F; j++) old[j] = k[j] != PCR_FB_CLEAR ? fb[pix[j]] : 0ull;
-#pragma unroll
-            for (int j = 0; j < F; j++)
-                if (k[j] < old[j]) {
-                    atomicMin(fb + pix[j], k[j]);
-                    lds_add32(natom, 1u);
-                }
-        }
+        tile_flush<BLOCK>(tile, fb, x0, y0, tw, n, row_magic(tw), (uint32_t)v.width, natom);
     } else {
         tile_dispatch<BLOCK, false, NT>(level, d, lo, med, hi, h, pj, fb, nullptr, 0, 0, 0, 0, natom);
     }
@@ -709,8 +721,8 @@ struct RasterPairView {
 };
 
 // The post-decode half of tile_point (interior batch, tiled) for one view.
-__device__ __forceinline__ void raster_pair_point(const float p[3], unsigned long long key_low, const RasterPairView& rv,
-                                                  uint32_t* natom) {
+__device__ __forceinline__ void raster_pair_point(const float p[3], unsigned long long key_low,
+                                                  const RasterPairView& rv, uint32_t* natom) {
     uint32_t ux, uy, depth;
     project_interior(rv.pj, p[0], p[1], p[2], rv.x0, rv.y0, ux, uy, depth);
     if (ux < (uint32_t)rv.tw && uy < (uint32_t)rv.th) {
@@ -793,35 +805,10 @@ __device__ __forceinline__ bool raster_pair_batch(uint32_t bi, int level, const 
     else raster_pair_loop<BLOCK, 2, NT>(d, lo, med, hi, h.first_point, h.num_points, a, c, natom);
     __syncthreads();
     // each view's flush, as in tiled_batch
-    const RasterPairView* vs[2] = {&a, &c};
-#pragma unroll
-    for (int e = 0; e < 2; e++) {
-        const RasterPairView& rv = *vs[e];
-        if (!rv.draw) continue;
-        const int n = rv.tw * rv.th;
-        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)rv.tw - 1) / (uint64_t)rv.tw);
-        const uint32_t width = (uint32_t)rv.pj.w;
-        constexpr int F = 4;
-        for (int i0 = threadIdx.x; i0 < n; i0 += F * BLOCK) {
-            unsigned long long k[F], old[F];
-            uint32_t pix[F];
-#pragma unroll
-            for (int j = 0; j < F; j++) {
-                const int i = i0 + j * BLOCK;
-                k[j] = i < n ? rv.tile[i] : PCR_FB_CLEAR;
-                const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)rv.tw;
-                pix[j] = (uint32_t)rv.x0 + ux + ((uint32_t)rv.y0 + uy) * width;
-            }
-#pragma unroll
-            for (int j = 0; j < F; j++) old[j] = k[j] != PCR_FB_CLEAR ? rv.fb[pix[j]] : 0ull;
-#pragma unroll
-            for (int j = 0; j < F; j++)
-                if (k[j] < old[j]) {
-                    atomicMin(rv.fb + pix[j], k[j]);
-                    lds_add32(natom, 1u);
-                }
-        }
-    }
+    if (a.draw)
+        tile_flush<BLOCK>(a.tile, a.fb, a.x0, a.y0, a.tw, a.tw * a.th, row_magic(a.tw), (uint32_t)a.pj.w, natom);
+    if (c.draw)
+        tile_flush<BLOCK>(c.tile, c.fb, c.x0, c.y0, c.tw, c.tw * c.th, row_magic(c.tw), (uint32_t)c.pj.w, natom);
     return true;
 }
 
@@ -981,7 +968,8 @@ __device__ __forceinline__ void hqs_band_point(uint32_t ux, uint32_t ry, uint32_
         lds_min32(tdepth + t, dep);
     } else {
         if (!(__uint_as_float(dep) <= __uint_as_float(tdepth[t]) * 1.01f)) return;
-        const unsigned long long rg = (unsigned long long)(color & 255u) | ((unsigned long long)((color >> 8) & 255u) << 32);
+        const unsigned long long rg =
+            (unsigned long long)(color & 255u) | ((unsigned long long)((color >> 8) & 255u) << 32);
         const unsigned long long bc = (unsigned long long)((color >> 16) & 255u) | (1ull << 32);
         if (t != pend.t) {
             pending_flush(pend, tacc);
@@ -1122,6 +1110,44 @@ struct HqsTile {
 // and each touched pixel is flushed once, in coalesced rows.
 constexpr int kHqsMaxBands = 8;
 
+// The tile loops of the hqs passes over the n pixels of a band, rows from
+// by0 of a rectangle at x0, tw pixels wide (row = i / tw as a multiply-high
+// by magic = row_magic(tw)). Pass 1 flushes each touched depth with
+// hqs_depth; pass 2 first loads the final depths and clears the sums, then
+// adds each touched pixel's sums to the accumulators (add_pairs).
+template <int BLOCK>
+__device__ __forceinline__ void hqs_depth_flush(const uint32_t* tdepth, uint32_t* __restrict__ depth, int x0,
+                                                uint32_t by0, int tw, int n, uint32_t magic, uint32_t width) {
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        const uint32_t t = tdepth[i];
+        if (t == PCR_DEPTH_CLEAR) continue;
+        const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
+        hqs_depth(depth, (uint32_t)x0 + ux + (by0 + uy) * width, t);
+    }
+}
+template <int BLOCK>
+__device__ __forceinline__ void hqs_sums_preload(uint32_t* tdepth, unsigned long long* tacc,
+                                                 const uint32_t* __restrict__ depth, int x0, uint32_t by0, int tw,
+                                                 int n, uint32_t magic, uint32_t width) {
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
+        tdepth[i] = depth[(uint32_t)x0 + ux + (by0 + uy) * width];
+        tacc[2 * i] = 0ull;
+        tacc[2 * i + 1] = 0ull;
+    }
+}
+template <int BLOCK>
+__device__ __forceinline__ void hqs_sums_flush(const unsigned long long* tacc, uint32_t* __restrict__ acc, int x0,
+                                               uint32_t by0, int tw, int n, uint32_t magic, uint32_t width) {
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        const unsigned long long rg = tacc[2 * i], bc = tacc[2 * i + 1];
+        if ((bc >> 32) == 0ull) continue;
+        const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
+        const uint32_t pix = (uint32_t)x0 + ux + (by0 + uy) * width;
+        add_pairs(acc + 4ull * pix, (uint32_t)rg, (uint32_t)(rg >> 32), (uint32_t)bc, (uint32_t)(bc >> 32));
+    }
+}
+
 // Returns the points skipped by the occlusion test (0 if the batch drew).
 template <int BLOCK, int TILE_PIX, int PASS, bool NT>
 __device__ __forceinline__ uint32_t hqs_tiled_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
@@ -1166,40 +1192,23 @@ __device__ __forceinline__ uint32_t hqs_tiled_batch(uint32_t bi, int level, cons
                                                   none);
         return 0;
     }
-    // row = i / tw as a multiply-high (see tiled_batch)
-    const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)tw - 1) / (uint64_t)tw);
+    const uint32_t magic = row_magic(tw), width = (uint32_t)v.width;
     for (int k = 0; k < nbands; k++) {
         const int boff = k * bh, rows = min(bh, th - boff);
         const int n = tw * rows;
         const uint32_t by0 = (uint32_t)(y0 + boff);
         if (k) __syncthreads();   // the previous band's flush has read the tile
-        for (int i = threadIdx.x; i < n; i += BLOCK) {
-            if (PASS == 0) {
-                tdepth[i] = PCR_DEPTH_CLEAR;
-            } else {
-                const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
-                tdepth[i] = depth[(uint32_t)x0 + ux + (by0 + uy) * (uint32_t)v.width];
-                tacc[2 * i] = 0ull;
-                tacc[2 * i + 1] = 0ull;
-            }
-        }
+        if (PASS == 0)
+            for (int i = threadIdx.x; i < n; i += BLOCK) tdepth[i] = PCR_DEPTH_CLEAR;
+        else
+            hqs_sums_preload<BLOCK>(tdepth, tacc, depth, x0, by0, tw, n, magic, width);
         __syncthreads();
         const TileBand tb{x0, y0, tw, th, boff, rows, k == 0};
         hqs_tile_dispatch<BLOCK, PASS, true, NT>(level, d, lo, med, hi, rgba, hp, pj, depth, acc, tdepth, tacc, tb,
                                                  interior);
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += BLOCK) {
-            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)tw;
-            const uint32_t pix = (uint32_t)x0 + ux + (by0 + uy) * (uint32_t)v.width;
-            if (PASS == 0) {
-                const uint32_t t = tdepth[i];
-                if (t != PCR_DEPTH_CLEAR) hqs_depth(depth, pix, t);
-            } else {
-                const unsigned long long rg = tacc[2 * i], bc = tacc[2 * i + 1];
-                if ((bc >> 32) != 0ull)
-                    add_pairs(acc + 4ull * pix, (uint32_t)rg, (uint32_t)(rg >> 32), (uint32_t)bc, (uint32_t)(bc >> 32));
-            }
-        }
+        if (PASS == 0) hqs_depth_flush<BLOCK>(tdepth, depth, x0, by0, tw, n, magic, width);
+        else hqs_sums_flush<BLOCK>(tacc, acc, x0, by0, tw, n, magic, width);
     }
     return 0;
 }
@@ -1309,22 +1318,13 @@ __device__ __forceinline__ bool hqs1_pair_batch(uint32_t bi, int level, const pc
     else if (level == 1) hqs1_pair_loop<BLOCK, 1, NT>(d, lo, med, hi, hp.first_point, hp.num_points, a, c);
     else hqs1_pair_loop<BLOCK, 2, NT>(d, lo, med, hi, hp.first_point, hp.num_points, a, c);
     __syncthreads();
-    // flush each view's touched pixels (row = i / tw as a multiply-high, see tiled_batch)
-    const PairView* vs[2] = {&a, &c};
-#pragma unroll
-    for (int e = 0; e < 2; e++) {
-        const PairView& pv = *vs[e];
-        if (!pv.draw) continue;
-        const int n = pv.tw * pv.th;
-        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)pv.tw - 1) / (uint64_t)pv.tw);
-        const uint32_t width = (uint32_t)pv.pj.w;
-        for (int i = threadIdx.x; i < n; i += BLOCK) {
-            const uint32_t t = pv.tdepth[i];
-            if (t == PCR_DEPTH_CLEAR) continue;
-            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)pv.tw;
-            hqs_depth(pv.depth, (uint32_t)pv.x0 + ux + ((uint32_t)pv.y0 + uy) * width, t);
-        }
-    }
+    // each view's flush, as in hqs_tiled_batch
+    if (a.draw)
+        hqs_depth_flush<BLOCK>(a.tdepth, a.depth, a.x0, (uint32_t)a.y0, a.tw, a.tw * a.th, row_magic(a.tw),
+                               (uint32_t)a.pj.w);
+    if (c.draw)
+        hqs_depth_flush<BLOCK>(c.tdepth, c.depth, c.x0, (uint32_t)c.y0, c.tw, c.tw * c.th, row_magic(c.tw),
+                               (uint32_t)c.pj.w);
     return true;
 }
 
@@ -1434,41 +1434,20 @@ __device__ __forceinline__ bool hqs2_pair_batch(uint32_t bi, int level, const pc
                       TileBand{ax0, ay0, atw, ath, 0, ath, true}, da};
     const Pair2View c{make_proj(vb.transform, vb.width, vb.height), depth_b, acc_b, tdepth + half, tacc + 2 * half,
                       TileBand{bx0, by0, btw, bth, 0, bth, true}, db};
-    const Pair2View* vs[2] = {&a, &c};
     // each drawing view's final depths into its half, its sums cleared
-#pragma unroll
-    for (int e = 0; e < 2; e++) {
-        const Pair2View& pv = *vs[e];
-        if (!pv.draw) continue;
-        const int n = pv.tb.tw * pv.tb.th;
-        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)pv.tb.tw - 1) / (uint64_t)pv.tb.tw);
-        for (int i = threadIdx.x; i < n; i += BLOCK) {
-            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)pv.tb.tw;
-            pv.tdepth[i] = pv.depth[(uint32_t)pv.tb.x0 + ux + ((uint32_t)pv.tb.y0 + uy) * (uint32_t)pv.pj.w];
-            pv.tacc[2 * i] = 0ull;
-            pv.tacc[2 * i + 1] = 0ull;
-        }
-    }
+    const uint32_t am = row_magic(atw), cm = row_magic(btw);
+    if (a.draw)
+        hqs_sums_preload<BLOCK>(a.tdepth, a.tacc, depth_a, ax0, (uint32_t)ay0, atw, atw * ath, am, (uint32_t)va.width);
+    if (c.draw)
+        hqs_sums_preload<BLOCK>(c.tdepth, c.tacc, depth_b, bx0, (uint32_t)by0, btw, btw * bth, cm, (uint32_t)vb.width);
     __syncthreads();
     if (level == 0) hqs2_pair_loop<BLOCK, 0, NT>(d, lo, med, hi, rgba, hp.first_point, hp.num_points, a, c);
     else if (level == 1) hqs2_pair_loop<BLOCK, 1, NT>(d, lo, med, hi, rgba, hp.first_point, hp.num_points, a, c);
     else hqs2_pair_loop<BLOCK, 2, NT>(d, lo, med, hi, rgba, hp.first_point, hp.num_points, a, c);
     __syncthreads();
-    // flush each view's touched pixels (row = i / tw as a multiply-high, see tiled_batch)
-#pragma unroll
-    for (int e = 0; e < 2; e++) {
-        const Pair2View& pv = *vs[e];
-        if (!pv.draw) continue;
-        const int n = pv.tb.tw * pv.tb.th;
-        const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)pv.tb.tw - 1) / (uint64_t)pv.tb.tw);
-        for (int i = threadIdx.x; i < n; i += BLOCK) {
-            const unsigned long long rg = pv.tacc[2 * i], bc = pv.tacc[2 * i + 1];
-            if ((bc >> 32) == 0ull) continue;
-            const uint32_t uy = __umulhi((uint32_t)i, magic), ux = (uint32_t)i - uy * (uint32_t)pv.tb.tw;
-            const uint32_t pix = (uint32_t)pv.tb.x0 + ux + ((uint32_t)pv.tb.y0 + uy) * (uint32_t)pv.pj.w;
-            add_pairs(pv.acc + 4ull * pix, (uint32_t)rg, (uint32_t)(rg >> 32), (uint32_t)bc, (uint32_t)(bc >> 32));
-        }
-    }
+    // each view's flush, as in hqs_tiled_batch
+    if (a.draw) hqs_sums_flush<BLOCK>(a.tacc, acc_a, ax0, (uint32_t)ay0, atw, atw * ath, am, (uint32_t)va.width);
+    if (c.draw) hqs_sums_flush<BLOCK>(c.tacc, acc_b, bx0, (uint32_t)by0, btw, btw * bth, cm, (uint32_t)vb.width);
     return true;
 }
 
