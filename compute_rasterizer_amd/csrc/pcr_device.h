@@ -346,6 +346,66 @@ __device__ inline bool screen_rect(const float* __restrict__ t, const float mn[3
     return x1 >= x0 && y1 >= y0;
 }
 
+// A batch's screen rectangle for one view, as screen_rect() reports it.
+struct RectInfo {
+    int x0, y0, x1, y1;
+    float wlo;
+    int interior;
+    int on;   // screen_rect()'s return value
+};
+
+// screen_rect() for up to eight views of one box at once: the lanes of each
+// 8-lane group (lane & ~7) project the corners for the group's own view (t,
+// w, h may differ between groups), one corner per lane, and reduce within the
+// group, so every lane of a group ends with that view's rectangle. The
+// operations per view are screen_rect()'s, with the ballots restricted to
+// the group and no early return, so the results are the same. Called with
+// whole groups active.
+__device__ inline void screen_rect_group(const float* __restrict__ t, const float mn[3], const float mx[3], int w,
+                                         int h, RectInfo& r) {
+    const uint32_t lane = __lane_id();
+    const unsigned long long gm = 0xFFull << (lane & ~7u);
+    const int c = (int)(lane & 7u);
+    const float x = (c & 1) ? mx[0] : mn[0], y = (c & 2) ? mx[1] : mn[1], z = (c & 4) ? mx[2] : mn[2];
+    const float px = fma_(t[8], z, fma_(t[4], y, fma_(t[0], x, t[12])));
+    const float py = fma_(t[9], z, fma_(t[5], y, fma_(t[1], x, t[13])));
+    const float pw = fma_(t[11], z, fma_(t[7], y, fma_(t[3], x, t[15])));
+    const bool front = (__ballot(!(pw > 0.0f)) & gm) == 0ull;
+    const float sx = fma_(px / pw, 0.5f, 0.5f) * (float)w, sy = fma_(py / pw, 0.5f, 0.5f) * (float)h;
+    float sx0 = sx, sx1 = sx, sy0 = sy, sy1 = sy;
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+        sx0 = fminf(sx0, __shfl_xor(sx0, m));
+        sx1 = fmaxf(sx1, __shfl_xor(sx1, m));
+        sy0 = fminf(sy0, __shfl_xor(sy0, m));
+        sy1 = fmaxf(sy1, __shfl_xor(sy1, m));
+    }
+    sx0 = fmaxf(sx0, -2.0f); sy0 = fmaxf(sy0, -2.0f);
+    sx1 = fminf(sx1, (float)w + 2.0f); sy1 = fminf(sy1, (float)h + 2.0f);
+    r.x0 = max((int)floorf(sx0) - 1, 0);
+    r.y0 = max((int)floorf(sy0) - 1, 0);
+    r.x1 = min((int)floorf(sx1) + 1, w - 1);
+    r.y1 = min((int)floorf(sy1) + 1, h - 1);
+    const float ax = fmaxf(fabsf(mn[0]), fabsf(mx[0])), ay = fmaxf(fabsf(mn[1]), fabsf(mx[1])),
+                az = fmaxf(fabsf(mn[2]), fabsf(mx[2]));
+    const float ex = fabsf(t[12]) + fabsf(t[0]) * ax + fabsf(t[4]) * ay + fabsf(t[8]) * az;
+    const float ey = fabsf(t[13]) + fabsf(t[1]) * ax + fabsf(t[5]) * ay + fabsf(t[9]) * az;
+    const float ew = fabsf(t[15]) + fabsf(t[3]) * ax + fabsf(t[7]) * ay + fabsf(t[11]) * az;
+    const bool ok = ex <= 0x1p38f && ey <= 0x1p38f && ew <= 0x1p38f && pw >= 0x1p-12f + 0x1p-16f * ew;
+    const bool interior = front && (__ballot(!ok) & gm) == 0ull && r.x0 >= 1 && r.y0 >= 1 && r.x1 <= w - 2 &&
+                          r.y1 <= h - 2;
+    float pwmin = pw;
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) pwmin = fminf(pwmin, __shfl_xor(pwmin, m));
+    r.wlo = interior && pwmin >= fmaxf((float)w, (float)h) * 0x1p-21f * (fmaxf(ex, ey) + ew) ? pwmin - 0x1p-18f * ew
+                                                                                            : 0.0f;
+    r.interior = interior;
+    r.on = front && r.x1 >= r.x0 && r.y1 >= r.y0;
+    if (!front) {   // screen_rect() returns before the rectangle: the caller's empty one
+        r.x0 = 0; r.y0 = 0; r.x1 = -1; r.y1 = -1;
+    }
+}
+
 // 64-bit fetch-min with the benign early-out compare of render.cs:423-426.
 // The plain load can only be stale-high within a frame (values only
 // decrease), so skipping when key >= old never loses a minimum. Returns 1
