@@ -546,49 +546,18 @@ __device__ __forceinline__ RectInfo uniform_rect(const RectInfo& s) {
     return r;
 }
 
-// tiled_batch() with the batch's header and its screen rectangle for the view
-// given, as the multi-view kernel computes them for all its views at once.
-// The steps after the rectangle are tiled_batch()'s. The single-view kernel
-// keeps its own copy: sharing this one changed how the compiler schedules
-// it, and its flythrough frames took 0.4% longer.
-template <int BLOCK, int TILE_PIX, bool NT>
-__device__ __forceinline__ uint32_t tiled_batch_r(pcr_batch h, int level, const uint32_t* __restrict__ lo,
-                                                  const uint32_t* __restrict__ med, const uint32_t* __restrict__ hi,
-                                                  const pcr_view& v, unsigned long long* __restrict__ fb,
-                                                  unsigned long long* tile, uint32_t* natom, bool occl, uint32_t part,
-                                                  uint32_t nparts, const RectInfo& r) {
-    if (nparts > 1) {
-        const uint32_t k = (h.num_points + nparts - 1) / nparts, off = min(part * k, h.num_points);
-        h.first_point += off;
-        h.num_points = min(k, h.num_points - off);
-    }
-    const int x0 = r.x0, y0 = r.y0, tw = r.x1 - r.x0 + 1, th = r.y1 - r.y0 + 1;
-    if (occl && r.wlo > 0.0f && tw >= 2 && tw * th <= kOcclusionMaxPix &&
-        !rect_open<BLOCK, 0>(fb, v.width, x0, y0, tw, tw * th, r.wlo))
-        return h.num_points;
-    const Decode d = make_decode(level, h.min, h.max);
-    const Proj pj = make_proj(v.transform, v.width, v.height);
-    if (r.on != 0 && tw >= 2 && tw <= TILE_PIX && th <= TILE_PIX && tw * th <= TILE_PIX) {
-        const int n = tw * th;
-        for (int i = threadIdx.x; i < n; i += BLOCK) tile[i] = PCR_FB_CLEAR;
-        __syncthreads();
-        tile_dispatch<BLOCK, true, NT>(level, d, lo, med, hi, h, pj, fb, tile, x0, y0, tw, th, natom, r.interior != 0);
-        __syncthreads();
-        tile_flush<BLOCK>(tile, fb, x0, y0, tw, n, row_magic(tw), (uint32_t)v.width, natom);
-    } else {
-        tile_dispatch<BLOCK, false, NT>(level, d, lo, med, hi, h, pj, fb, nullptr, 0, 0, 0, 0, natom);
-    }
-    return 0;
-}
-
 // One batch of the tiled raster; `bi` and `level` are workgroup-uniform.
 // Returns the points skipped by the occlusion test (0 if the batch drew).
-template <int BLOCK, int TILE_PIX, bool NT>
+// PRE: the batch's screen rectangle for the view comes in `pre` (the
+// multi-view kernel computes it for all its views at once) instead of from
+// screen_rect() here.
+template <int BLOCK, int TILE_PIX, bool NT, bool PRE = false>
 __device__ __forceinline__ uint32_t tiled_batch(uint32_t bi, int level, const pcr_batch* __restrict__ b,
                                                 const uint32_t* __restrict__ lo, const uint32_t* __restrict__ med,
                                                 const uint32_t* __restrict__ hi, const pcr_view& v,
                                                 unsigned long long* __restrict__ fb, unsigned long long* tile,
-                                                uint32_t* natom, bool occl, uint32_t part = 0, uint32_t nparts = 1) {
+                                                uint32_t* natom, bool occl, uint32_t part = 0, uint32_t nparts = 1,
+                                                const RectInfo* pre = nullptr) {
     pcr_batch h = b[bi];
     if (nparts > 1) {   // points [part * k, (part + 1) * k) of the batch, k = ceil(n / nparts)
         const uint32_t k = (h.num_points + nparts - 1) / nparts, off = min(part * k, h.num_points);
@@ -596,9 +565,16 @@ __device__ __forceinline__ uint32_t tiled_batch(uint32_t bi, int level, const pc
         h.num_points = min(k, h.num_points - off);
     }
     int x0 = 0, y0 = 0, x1 = -1, y1 = -1;
-    bool interior;
+    bool interior, use;
     float wlo;
-    bool use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1, &interior, &wlo);
+    if constexpr (PRE) {
+        x0 = pre->x0; y0 = pre->y0; x1 = pre->x1; y1 = pre->y1;
+        interior = pre->interior != 0;
+        wlo = pre->wlo;
+        use = pre->on != 0;
+    } else {
+        use = screen_rect(v.transform, h.min, h.max, v.width, v.height, x0, y0, x1, y1, &interior, &wlo);
+    }
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
     // a batch behind everything already drawn over its rectangle draws nothing
     // (wlo > 0 only for interior batches, whose rectangle is not empty); the
@@ -924,8 +900,8 @@ __global__ __launch_bounds__(BLOCK) void k_mv_raster_tiled(const pcr_batch* __re
                     continue;
                 }
             }
-            tiled_batch_r<BLOCK, TILE_PIX, false>(b[bi], lv, lo, med, hi, views[vi], fbs[vi], tile, &s_natom,
-                                                  occl != 0, part, nparts, ra);
+            tiled_batch<BLOCK, TILE_PIX, false, true>(bi, lv, b, lo, med, hi, views[vi], fbs[vi], tile, &s_natom,
+                                                      occl != 0, part, nparts, &ra);
             __syncthreads();   // the tile is reused by the next view
             k++;
         }
